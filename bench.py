@@ -1,0 +1,181 @@
+"""Benchmark of the Born RTM hot path on one MI355X.
+
+A step is one full RTM shot of the 512^3 production workload (configs.rtm512:
+512^3 FP32 grid, d = 12.5 m, nt = 4000): source propagation with FP32
+checkpoints every 50 steps, then the adjoint (receiver-field) propagation
+with segment recompute of the source field and fused zero-lag imaging.
+Inputs (velocity, source, Born-modelled receiver data) are resident in HBM
+before the timed region.
+
+value = RTM Gpts/s = 2 * n1*n2*n3 * nt / time per shot (the two wavefields
+an RTM must advance; the checkpoint recompute is overhead, not credited).
+
+roofline: the dominant kernel is the fused VL stencil step (k_step, no
+imaging; it runs for the source propagation and the recompute).  Its launches
+inside the timed region are bracketed by HIP events on the library stream;
+achieved = 16 B per cell (read p_n, p_{n-1}, V; write p_{n+1}) * n1*n2*n3 /
+mean launch duration.
+
+cpu_baseline: the FP32 OpenMP CPU restatement (oracle/liborc32.so) of the
+same stencil step at 512^3, timed on the host cores for ~10 s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 16.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup_device(w, ty, target):
+    from sep_code_amd.device import BornDevice
+    dev = BornDevice(0)
+    dev.set_tuning(ty, target)
+    shot = w.shots[0]
+    sinc = np.zeros((1, 8), np.float32)
+    sinc[0, 3] = 1
+    dev.create_space(w.d, w.bc_a, w.bc_b, w.n)
+    dev.set_velocity(w.V)
+    dev.set_imaging_scale(w.cimg)
+    dev.set_sinc_table(0, sinc)
+    dev.set_sinc_table(1, sinc)
+    dev.set_source(shot.src_locs, shot.src_traces)
+    dev.set_receivers(shot.rec_locs, None, ntr=w.ntr, nx=shot.nrx, ny=shot.nry)
+    # observed data: Born modelling of the true reflectivity (source field
+    # propagated alongside), left resident in the receiver buffer.
+    dev.select_shot(0)
+    dev.source_prop(w.jt, w.nt, w.jimg, store="none")
+    dev.set_model(w.refl)
+    dev.rtm_forward(w.jt, w.nt, w.jimg)
+    dev.sync()
+    return dev
+
+
+def rtm_shot(dev, w):
+    dev.source_prop(w.jt, w.nt, w.jimg, store=w.store, ckpt=w.ckpt)
+    dev.zero_image()
+    dev.rtm_adjoint(w.jt, w.nt, w.jimg)
+
+
+def cpu_baseline(w, seconds=10.0):
+    """FP32 OpenMP restatement of the VL step at the workload's size."""
+    from oracle import born_oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    P = O.Problem(w.n, w.d, w.V, w.bc_a, w.bc_b, prec=32)
+    rng = np.random.default_rng(0)
+    cur = np.zeros(w.ncell, np.float32)
+    old = np.zeros(w.ncell, np.float32)
+    v = cur.reshape(w.n[2], w.n[1], w.n[0])
+    v[4:-4, 4:-4, 4:-4] = rng.standard_normal((w.n[2] - 8, w.n[1] - 8, w.n[0] - 8))
+    import ctypes as C
+    P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)
+        steps += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and steps >= 3) or steps >= 1000:
+            break
+    gpts = w.ncell * steps / el / 1e9
+    return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} FP32 VL stencil steps (fused sponge) of the {w.n[0]}x{w.n[1]}x"
+                      f"{w.n[2]} grid on {threads} OpenMP threads, {el:.1f} s; "
+                      "per-step rate, not the full 4000-step shot"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rtm512")
+    ap.add_argument("--nt", type=int, default=0, help="override nt (testing only)")
+    ap.add_argument("--ty", type=int, default=16)
+    ap.add_argument("--target-blocks", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--traffic-bytes", type=float, default=0.0,
+                    help="HBM bytes per stencil launch measured with rocprofv3 --pmc")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the whole shot domain "
+                                   "sits in one MI355X's HBM; multi-GPU sharding is by shot "
+                                   "and is not part of this build", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    from sep_code_amd import configs
+    t0 = time.perf_counter()
+    kw = {"nt": a.nt} if a.nt else {}
+    w = configs.WORKLOADS[a.workload](**kw)
+    log(f"workload {w.name} n={w.n} nt={w.nt} generated in {time.perf_counter() - t0:.1f}s")
+    dev = setup_device(w, a.ty, a.target_blocks)
+    log(f"setup done ({dev.device_bytes() / 2**30:.1f} GiB device) "
+        f"{time.perf_counter() - t0:.1f}s")
+
+    for i in range(a.warmup):
+        rtm_shot(dev, w)
+        dev.sync()
+        log(f"warmup {i + 1}/{a.warmup} done")
+
+    dev.kernel_timing(True, 0, 0)
+    dev.sync()
+    t1 = time.perf_counter()
+    for i in range(a.steps):
+        rtm_shot(dev, w)
+        dev.sync()
+        log(f"step {i + 1}/{a.steps} done at {time.perf_counter() - t1:.2f}s")
+    dev.sync()
+    el = time.perf_counter() - t1
+    ktot, kcount = dev.kernel_timing_result()
+    dev.kernel_timing(False)
+    img = dev.get_image()
+    finite = bool(np.isfinite(img).all())
+
+    ncell = w.ncell
+    ms_step = el / a.steps * 1e3
+    value = 2.0 * ncell * w.nt * a.steps / el / 1e9
+    k_ms = ktot / max(kcount, 1)
+    achieved = BYTES_PER_CELL * ncell / (k_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": (a.traffic_bytes or None),
+            "kernel": "k_step<VL,fused sponge> (brtm::k_step FORM_VL FMT_NONE)",
+            "kernel_ms": round(k_ms, 4), "launches": kcount,
+            "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(w, a.cpu_seconds)
+    out = {
+        "metric": "RTM Gpts/s (% HBM roofline); LSRTM time-to-1e-2 residual, learned vs plain CG",
+        "value": round(value, 3), "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{w.name}: RTM shot, {w.n[0]}x{w.n[1]}x{w.n[2]} FP32, "
+                               f"nt={w.nt}, source checkpoints every {w.ckpt} steps, "
+                               "imaging every step",
+                   "grid": list(w.n), "nt": w.nt, "store": w.store, "ckpt": w.ckpt,
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "image_finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

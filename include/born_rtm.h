@@ -1,0 +1,170 @@
+/*
+ * born_rtm.h -- C ABI of the MI355X Born acoustic RTM library
+ * (sep_code_amd/lib/libborn_rtm.so).
+ *
+ * This is the drop-in boundary for the reference's device layer
+ * Born/gpu_funcs_3d.h:5-27.  Every entry point below names the reference
+ * function it replaces.  Differences from the reference, by design:
+ *   - state lives in an explicit context (the reference keeps file-scope
+ *     globals and __constant__ symbols, Born/gpu_finite_3d.h:31-81);
+ *   - every call returns an int status (BRTM_OK = 0) and brtm_last_error()
+ *     describes the last failure of the calling thread (the reference
+ *     returns void and mostly ignores errors, Born/gpu_funcs_3d.cu:47-62);
+ *   - device buffers are owned by the context and freed only by
+ *     brtm_destroy / brtm_release_store (the reference frees traces and the
+ *     source inside each propagation, Born/gpu_funcs_3d.cu:312-313,500-503);
+ *   - one GPU holds the whole domain: the slab split and P2P halos of the
+ *     reference (Born/gpu_funcs_3d.cu:143,256-269) are plain in-HBM padding.
+ * A context must be used by one host thread at a time.
+ *
+ * Host arrays are caller-owned, FP32, little endian.  Volumes are
+ * [n3][n2][n1] with n1 fastest on the padded compute grid; the stencil
+ * radius is 4 and cells within 4 of a face are never updated.  Receiver
+ * traces cross the boundary in the reference layout [ny][nx][ntr] (time
+ * fastest); inside the library they are stored [ntr][nrec] so a time step
+ * touches one contiguous row.
+ *
+ * The reference's exact signatures (setup_cuda, source_prop, rtm_forward,
+ * rtm_adjoint, ...) are provided over a process-wide default context by
+ * gpu_funcs_3d.h in this directory.
+ */
+#ifndef BORN_RTM_H
+#define BORN_RTM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRTM_OK 0
+#define BRTM_EINVAL -1     /* invalid argument                          */
+#define BRTM_EHIP -2       /* HIP runtime error                         */
+#define BRTM_ENOMEM -3     /* device or host allocation failed          */
+#define BRTM_ESTATE -4     /* call order violated (e.g. no velocity)    */
+#define BRTM_EUNSTABLE -5  /* velocity violates the leapfrog CFL limit  */
+
+/* Source-wavefield policies for brtm_source_prop. */
+#define BRTM_STORE_NONE 0  /* keep nothing (last two fields only)             */
+#define BRTM_STORE_F32 1   /* snapshot every jimg steps, FP32                 */
+#define BRTM_STORE_BF16 2  /* snapshot every jimg steps, BF16 + per-z scale   */
+#define BRTM_STORE_F16 3   /* snapshot every jimg steps, FP16 + per-z scale   */
+#define BRTM_STORE_CKPT 4  /* FP32 checkpoints every ckpt steps, recompute    */
+
+typedef struct brtm_ctx brtm_ctx;
+
+/* Message describing the last failing call on this thread. */
+const char *brtm_last_error(void);
+
+/* Library version string. */
+const char *brtm_version(void);
+
+/* Replaces setup_cuda (Born/gpu_funcs_3d.cu:10-45): bind a context to one
+ * GPU (no peer access: one GPU holds the domain). */
+int brtm_create(int device, brtm_ctx **out);
+int brtm_destroy(brtm_ctx *ctx);
+
+/* Replaces create_gpu_space (Born/gpu_funcs_3d.cu:777-803): grid spacing,
+ * sponge parameters (w = exp(-bc_b (bc_a - e)) for edge distance e < bc_a,
+ * Born/wave_fkernel.3d8o.cu:194-205) and compute-grid size.  Allocates the
+ * wavefields.  Stencil weights are formed in double as
+ * Born/wave_fkernel.3d8o.cu:26-27. */
+int brtm_create_space(brtm_ctx *ctx, float d1, float d2, float d3, float bc_a, float bc_b,
+                      int n1, int n2, int n3);
+
+/* Replaces transfer_vel_func1 (Born/gpu_funcs_3d.cu:761-767): V = v^2 dt^2
+ * on the compute grid.  Returns BRTM_EUNSTABLE when max V exceeds the 3-D
+ * 8th-order leapfrog limit (the reference's 0.5 dmin/vmax rule at
+ * Born/fd_prop_3d.cpp:13 does not guarantee it). */
+int brtm_set_velocity(brtm_ctx *ctx, const float *V);
+
+/* Replaces transfer_sinc_table_s / _d (Born/gpu_funcs_3d.cu:714-733):
+ * which = 0 source table, 1 receiver table; table is [ntab][nsinc], nsinc
+ * must be 8; ntab = steps per data sample (jt). */
+int brtm_set_sinc_table(brtm_ctx *ctx, int which, int nsinc, int ntab, const float *table);
+
+/* Replaces transfer_source_func (Born/gpu_funcs_3d.cu:735-745): npts
+ * injection cells (linear index on the compute grid, inside the update
+ * set) and their traces [npts][ntr]. */
+int brtm_set_source(brtm_ctx *ctx, int npts, int ntr, const int *locs, const float *vals);
+
+/* Replaces transfer_receiver_func (Born/gpu_funcs_3d.cu:747-758): nx*ny
+ * receiver cells (locs < 0, or cells outside the update set, are inactive)
+ * and their traces [ny][nx][ntr] (vals may be NULL: zero traces). */
+int brtm_set_receivers(brtm_ctx *ctx, int nx, int ny, int ntr, const int *locs,
+                       const float *vals);
+/* Download the receiver traces in the layout [ny][nx][ntr]. */
+int brtm_get_receivers(brtm_ctx *ctx, float *vals);
+
+/* Born model (compute grid, the img argument of rtm_forward,
+ * Born/gpu_funcs_3d.cu:317,381) and image accumulator (the img output of
+ * rtm_adjoint, :509,695). */
+int brtm_set_model(brtm_ctx *ctx, const float *m);
+int brtm_get_image(brtm_ctx *ctx, float *img);
+int brtm_zero_image(brtm_ctx *ctx);
+
+/* Source-wavefield stores are kept per shot; subsequent propagation calls
+ * use the store of the selected shot. */
+int brtm_select_shot(brtm_ctx *ctx, int ishot);
+int brtm_release_store(brtm_ctx *ctx, int ishot);
+
+/* Replaces source_prop (Born/gpu_funcs_3d.cu:133-315): nt steps of the
+ * source field s^{it+1} = w.(V.L s^it + 2 s^it) - w^2 s^{it-1} plus
+ * injection (dir = +1, sinc row it % jt at sample it / jt).  store selects
+ * what is kept for the selected shot: snapshots s^{it+1} for it % jimg == 0
+ * (FP32/BF16/FP16), or FP32 checkpoints every ckpt steps.  p0, p1 (may be
+ * NULL) receive s^{nt-1} and s^{nt}. */
+int brtm_source_prop(brtm_ctx *ctx, int jt, int nt, int jimg, int store, int ckpt, float *p0,
+                     float *p1);
+
+/* Replaces rtm_forward (Born/gpu_funcs_3d.cu:317-505): Born modelling of
+ * the model set by brtm_set_model into the receiver traces (zeroed first
+ * unless accumulate).  Scattered field q^{it+1} = L(V.w.q^it) + 2w q^it -
+ * w^2 q^{it-1} + cimg s^{it+1}.m (Born source when it % jimg == 0), then
+ * extraction at sample it/jt + k with receiver sinc row it % jt.  The source
+ * field comes from the selected shot's store, or is propagated alongside
+ * when the store is BRTM_STORE_NONE / BRTM_STORE_CKPT. */
+int brtm_rtm_forward(brtm_ctx *ctx, int jt, int nt, int jimg, int accumulate);
+
+/* Replaces rtm_adjoint (Born/gpu_funcs_3d.cu:509-712): the exact transpose
+ * of brtm_rtm_forward.  Receiver field lambda^{it+1} = w.(V.L lambda^{it+2}
+ * + 2 lambda^{it+2}) - w^2 lambda^{it+3} + injected traces, image +=
+ * cimg s^{it+1}.lambda^{it+1} when it % jimg == 0.  The image accumulator
+ * is added to (brtm_zero_image clears it).  Requires a store for the
+ * selected shot (snapshots or checkpoints). */
+int brtm_rtm_adjoint(brtm_ctx *ctx, int jt, int nt, int jimg);
+
+/* Imaging scale (1e-6 in the reference, Born/gpu_funcs_3d.cu:117,128). */
+int brtm_set_imaging_scale(brtm_ctx *ctx, float cimg);
+
+/* Decode snapshot k of the selected shot's store into host FP32. */
+int brtm_get_snapshot(brtm_ctx *ctx, int k, float *out);
+
+/* Plain propagation of host fields (tests and timing): nsteps of the VL
+ * (form 0) or LV (form 1) step, no sources; p_cur/p_old are replaced by the
+ * newest and previous fields. */
+int brtm_propagate(brtm_ctx *ctx, int form, int nsteps, float *p_cur, float *p_old);
+
+/* Kernel timing: run nrep steps of the stencil (form 0/1, fmt 0..3 fused
+ * imaging/Born-source format) on the context's resident fields between two
+ * HIP events on the library stream; avg_ms receives the mean duration. */
+int brtm_time_step(brtm_ctx *ctx, int form, int fmt, int nrep, double *avg_ms);
+
+/* Per-launch timing of one stencil variant (form 0/1, fmt 0..3): while
+ * enabled, every launch of that variant is bracketed by HIP events on the
+ * library stream; the result is the summed duration and launch count since
+ * the last brtm_kernel_timing call. */
+int brtm_kernel_timing(brtm_ctx *ctx, int enable, int form, int fmt);
+int brtm_kernel_timing_result(brtm_ctx *ctx, double *total_ms, int *count);
+
+/* Tile rows per workgroup (8, 16 or 32) and target workgroup count. */
+int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
+
+/* Device synchronisation of the library stream. */
+int brtm_sync(brtm_ctx *ctx);
+
+/* Bytes of device memory the context currently holds. */
+int brtm_device_bytes(brtm_ctx *ctx, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BORN_RTM_H */

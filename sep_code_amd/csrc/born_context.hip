@@ -1,0 +1,750 @@
+// Context, propagation loops and C ABI of the Born RTM library.
+//
+// The time loops restate Born/gpu_funcs_3d.cu: source_prop (:133-315),
+// rtm_forward (:317-505) and rtm_adjoint (:509-712), on one GPU, one
+// stream, with the sponge, imaging and Born source fused into the stencil
+// launch and no per-step device synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/born_rtm.h"
+#include "born_ctx.h"
+
+namespace brtm {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int DevBuf::alloc(size_t n) {
+  if (ptr && bytes >= n) return BRTM_OK;
+  release();
+  if (n == 0) return BRTM_OK;
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e));
+    return BRTM_ENOMEM;
+  }
+  bytes = n;
+  return BRTM_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+Store &cur_store(brtm_ctx *c) {
+  if ((int)c->stores.size() <= c->shot) c->stores.resize(c->shot + 1);
+  return c->stores[c->shot];
+}
+
+static int fmt_of_store(int kind) {
+  switch (kind) {
+    case BRTM_STORE_F32: return FMT_F32;
+    case BRTM_STORE_BF16: return FMT_BF16;
+    case BRTM_STORE_F16: return FMT_F16;
+    default: return FMT_NONE;
+  }
+}
+
+static size_t fmt_bytes(int fmt) { return fmt == FMT_F32 ? 4 : 2; }
+
+int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float *pn,
+         const void *snap, const float *sscale, float *img, const float *m) {
+  StepParams P{};
+  P.n1 = c->n1;
+  P.n2 = c->n2;
+  P.n3 = c->n3;
+  P.nw = c->nw;
+  std::memcpy(P.c, c->coeffs, sizeof(P.c));
+  P.p = p;
+  P.po = po;
+  P.pn = pn;
+  P.V = c->V.as<float>();
+  P.wtab = c->d_wtab.as<float>();
+  P.img = img;
+  P.snap = snap;
+  P.sscale = sscale;
+  P.m = m;
+  P.cimg = c->cimg;
+  plan_step(P, c->ty_tile, c->target_blocks);
+  const bool timed = c->ktime_on && form == c->ktime_form && fmt == c->ktime_fmt;
+  if (timed) {
+    if (c->ktime_used + 2 > c->ktime_ev.size()) {
+      for (int k = 0; k < 256; ++k) {
+        hipEvent_t e;
+        BRTM_HIP(hipEventCreate(&e));
+        c->ktime_ev.push_back(e);
+      }
+    }
+    BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
+  }
+  BRTM_HIP(launch_step(form, fmt, P, c->ty_tile, c->stream));
+  if (timed) {
+    BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
+    c->ktime_used += 2;
+  }
+  return BRTM_OK;
+}
+
+static int inject_source(brtm_ctx *c, int it, int jt, float *p) {
+  if (c->nsrc == 0) return BRTM_OK;
+  InjectParams I{};
+  I.npts = c->nsrc;
+  I.locs = c->slocs.as<int>();
+  I.tr = c->strc.as<float>();
+  I.sp = c->ntr_s;
+  I.st = 1;
+  I.id = it / jt;
+  std::memcpy(I.sinc8, &c->sinc[0][8 * (it % jt)], sizeof(I.sinc8));
+  I.scale = 1.f;
+  I.p = p;
+  I.n12 = c->n1 * c->n2;
+  BRTM_HIP(launch_inject(I, c->stream));
+  return BRTM_OK;
+}
+
+static int inject_data(brtm_ctx *c, int it, int jt, float *p, const void *snap, int fmt,
+                       const float *sscale, float *img) {
+  if (c->nrec == 0) return BRTM_OK;
+  InjectParams I{};
+  I.npts = c->nrec;
+  I.locs = c->rlocs.as<int>();
+  I.tr = c->traces.as<float>();
+  I.sp = 1;
+  I.st = c->nrec;
+  I.id = it / jt;
+  std::memcpy(I.sinc8, &c->sinc[1][8 * (it % jt)], sizeof(I.sinc8));
+  I.scale = 1.f;
+  I.p = p;
+  I.img = img;
+  I.snap = snap;
+  I.sscale = sscale;
+  I.fmt = fmt;
+  I.n12 = c->n1 * c->n2;
+  I.cimg = c->cimg;
+  BRTM_HIP(launch_inject(I, c->stream));
+  return BRTM_OK;
+}
+
+static int extract_data(brtm_ctx *c, int it, int jt, const float *p) {
+  if (c->nrec == 0) return BRTM_OK;
+  BRTM_HIP(launch_extract(c->nrec, c->rlocs.as<int>(), c->traces.as<float>(), c->nrec, it / jt,
+                          &c->sinc[1][8 * (it % jt)], p, c->stream));
+  return BRTM_OK;
+}
+
+static int check_ready(brtm_ctx *c, int jt, int nt, int jimg) {
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_REQUIRE(c->have_vel, BRTM_ESTATE, "brtm_set_velocity has not been called");
+  BRTM_REQUIRE(jt >= 1 && nt >= 1 && jimg >= 1, BRTM_EINVAL, "jt, nt and jimg must be >= 1");
+  BRTM_REQUIRE((int)c->sinc[0].size() >= 8 * jt && (int)c->sinc[1].size() >= 8 * jt,
+               BRTM_ESTATE, "interpolation tables have fewer than jt rows");
+  return BRTM_OK;
+}
+
+#define BRTM_TRY(x)              \
+  do {                           \
+    int rc_ = (x);               \
+    if (rc_ != BRTM_OK) return rc_; \
+  } while (0)
+
+int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float *p0,
+                float *p1) {
+  BRTM_TRY(check_ready(c, jt, nt, jimg));
+  BRTM_REQUIRE(kind >= BRTM_STORE_NONE && kind <= BRTM_STORE_CKPT, BRTM_EINVAL,
+               "unknown store kind");
+  BRTM_REQUIRE(kind != BRTM_STORE_CKPT || ckpt >= 1, BRTM_EINVAL, "ckpt must be >= 1");
+  BRTM_REQUIRE(c->nsrc == 0 || (nt - 1) / jt + 8 <= c->ntr_s, BRTM_EINVAL,
+               "source traces shorter than (nt-1)/jt + 8 samples");
+  Store &S = cur_store(c);
+  S.data.release();
+  S.scale.release();
+  S.kind = kind;
+  S.nt = nt;
+  S.jt = jt;
+  S.jimg = jimg;
+  S.ckpt = ckpt;
+  S.count = 0;
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  const int fmt = fmt_of_store(kind);
+  if (fmt != FMT_NONE) {
+    S.count = (nt - 1) / jimg + 1;
+    BRTM_TRY(S.data.alloc(fmt_bytes(fmt) * (size_t)c->ncell * S.count));
+    if (fmt != FMT_F32) BRTM_TRY(S.scale.alloc(sizeof(float) * (size_t)c->n3 * S.count));
+  } else if (kind == BRTM_STORE_CKPT) {
+    S.count = (nt + ckpt - 1) / ckpt;
+    BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
+  }
+  float *cur = c->fb.as<float>(), *old = c->fa.as<float>();
+  BRTM_HIP(hipMemsetAsync(cur, 0, fbytes, c->stream));
+  BRTM_HIP(hipMemsetAsync(old, 0, fbytes, c->stream));
+  const int n12 = c->n1 * c->n2;
+  for (int it = 0; it < nt; ++it) {
+    if (kind == BRTM_STORE_CKPT && it % ckpt == 0) {
+      char *base = S.data.as<char>() + 2 * fbytes * (size_t)(it / ckpt);
+      BRTM_HIP(hipMemcpyAsync(base, cur, fbytes, hipMemcpyDeviceToDevice, c->stream));
+      BRTM_HIP(hipMemcpyAsync(base + fbytes, old, fbytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    BRTM_TRY(step(c, FORM_VL, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr, nullptr));
+    BRTM_TRY(inject_source(c, it, jt, old));
+    if (fmt != FMT_NONE && it % jimg == 0) {
+      const int k = it / jimg;
+      if (fmt == FMT_F32) {
+        BRTM_HIP(hipMemcpyAsync(S.data.as<char>() + fbytes * (size_t)k, old, fbytes,
+                                hipMemcpyDeviceToDevice, c->stream));
+      } else {
+        BRTM_HIP(launch_encode(fmt, old, S.data.as<char>() + 2 * (size_t)c->ncell * k,
+                               S.scale.as<float>() + (size_t)c->n3 * k, n12, c->n3, c->stream));
+      }
+    }
+    std::swap(cur, old);
+  }
+  if (p0) BRTM_HIP(hipMemcpyAsync(p0, old, fbytes, hipMemcpyDeviceToHost, c->stream));
+  if (p1) BRTM_HIP(hipMemcpyAsync(p1, cur, fbytes, hipMemcpyDeviceToHost, c->stream));
+  if (p0 || p1) BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
+  BRTM_TRY(check_ready(c, jt, nt, jimg));
+  BRTM_REQUIRE(c->nrec == 0 || (nt - 1) / jt + 8 <= c->ntr_d, BRTM_EINVAL,
+               "receiver traces shorter than (nt-1)/jt + 8 samples");
+  Store &S = cur_store(c);
+  const int sfmt = fmt_of_store(S.kind);
+  const bool live = sfmt == FMT_NONE;
+  if (!live) {
+    BRTM_REQUIRE(S.nt == nt && S.jimg == jimg && S.jt == jt, BRTM_ESTATE,
+                 "stored source wavefield was made with other nt/jt/jimg");
+  }
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  if (!accumulate && c->nrec)
+    BRTM_HIP(hipMemsetAsync(c->traces.ptr, 0, sizeof(float) * (size_t)c->nrec * c->ntr_d,
+                            c->stream));
+  float *qc = c->fd.as<float>(), *qo = c->fc.as<float>();
+  float *sc = c->fb.as<float>(), *so = c->fa.as<float>();
+  BRTM_HIP(hipMemsetAsync(qc, 0, fbytes, c->stream));
+  BRTM_HIP(hipMemsetAsync(qo, 0, fbytes, c->stream));
+  if (live) {
+    BRTM_HIP(hipMemsetAsync(sc, 0, fbytes, c->stream));
+    BRTM_HIP(hipMemsetAsync(so, 0, fbytes, c->stream));
+  }
+  for (int it = 0; it < nt; ++it) {
+    if (live) {
+      BRTM_TRY(step(c, FORM_VL, FMT_NONE, sc, so, so, nullptr, nullptr, nullptr, nullptr));
+      BRTM_TRY(inject_source(c, it, jt, so));
+    }
+    if (it % jimg == 0) {
+      const int k = it / jimg;
+      const void *snap;
+      const float *ss = nullptr;
+      int f;
+      if (live) {
+        snap = so;
+        f = FMT_F32;
+      } else {
+        f = sfmt;
+        snap = S.data.as<char>() + fmt_bytes(f) * (size_t)c->ncell * k;
+        if (f != FMT_F32) ss = S.scale.as<float>() + (size_t)c->n3 * k;
+      }
+      BRTM_TRY(step(c, FORM_LV, f, qc, qo, qo, snap, ss, nullptr, c->mdl.as<float>()));
+    } else {
+      BRTM_TRY(step(c, FORM_LV, FMT_NONE, qc, qo, qo, nullptr, nullptr, nullptr, nullptr));
+    }
+    BRTM_TRY(extract_data(c, it, jt, qo));
+    std::swap(qc, qo);
+    if (live) std::swap(sc, so);
+  }
+  return BRTM_OK;
+}
+
+int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
+  BRTM_TRY(check_ready(c, jt, nt, jimg));
+  BRTM_REQUIRE(c->nrec == 0 || (nt - 1) / jt + 8 <= c->ntr_d, BRTM_EINVAL,
+               "receiver traces shorter than (nt-1)/jt + 8 samples");
+  Store &S = cur_store(c);
+  BRTM_REQUIRE(S.kind != BRTM_STORE_NONE, BRTM_ESTATE,
+               "rtm_adjoint needs a stored source wavefield (brtm_source_prop with a store)");
+  BRTM_REQUIRE(S.nt == nt && S.jimg == jimg && S.jt == jt, BRTM_ESTATE,
+               "stored source wavefield was made with other nt/jt/jimg");
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  float *lc = c->fd.as<float>(), *lo = c->fc.as<float>();
+  BRTM_HIP(hipMemsetAsync(lc, 0, fbytes, c->stream));
+  BRTM_HIP(hipMemsetAsync(lo, 0, fbytes, c->stream));
+  float *img = c->img.as<float>();
+  const int sfmt = fmt_of_store(S.kind);
+  if (sfmt != FMT_NONE) {
+    for (int it = nt - 1; it >= 0; --it) {
+      if (it % jimg == 0) {
+        const int k = it / jimg;
+        const void *snap = S.data.as<char>() + fmt_bytes(sfmt) * (size_t)c->ncell * k;
+        const float *ss = sfmt == FMT_F32 ? nullptr : S.scale.as<float>() + (size_t)c->n3 * k;
+        BRTM_TRY(step(c, FORM_VL, sfmt, lc, lo, lo, snap, ss, img, nullptr));
+        BRTM_TRY(inject_data(c, it, jt, lo, snap, sfmt, ss, img));
+      } else {
+        BRTM_TRY(step(c, FORM_VL, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr, nullptr));
+        BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
+      }
+      std::swap(lc, lo);
+    }
+    return BRTM_OK;
+  }
+  // Checkpoints: recompute each segment of the source field out of place
+  // into seg slots 0..ckpt+1 (slot j holds s^{k0-1+j}), then step the
+  // receiver field back through the segment.
+  const int ck = S.ckpt;
+  if (c->seg_slots < ck + 2) {
+    BRTM_TRY(c->seg.alloc(fbytes * (size_t)(ck + 2)));
+    c->seg_slots = ck + 2;
+  }
+  auto slot = [&](int j) { return c->seg.as<float>() + (size_t)c->ncell * j; };
+  for (int cidx = S.count - 1; cidx >= 0; --cidx) {
+    const int k0 = cidx * ck, k1 = std::min(k0 + ck, nt);
+    const char *base = S.data.as<char>() + 2 * fbytes * (size_t)cidx;
+    BRTM_HIP(hipMemcpyAsync(slot(1), base, fbytes, hipMemcpyDeviceToDevice, c->stream));
+    BRTM_HIP(hipMemcpyAsync(slot(0), base + fbytes, fbytes, hipMemcpyDeviceToDevice, c->stream));
+    for (int it = k0; it < k1; ++it) {
+      const int j = it - k0;
+      BRTM_TRY(step(c, FORM_VL, FMT_NONE, slot(j + 1), slot(j), slot(j + 2), nullptr, nullptr,
+                    nullptr, nullptr));
+      BRTM_TRY(inject_source(c, it, jt, slot(j + 2)));
+    }
+    for (int it = k1 - 1; it >= k0; --it) {
+      if (it % jimg == 0) {
+        const float *snap = slot(it - k0 + 2);
+        BRTM_TRY(step(c, FORM_VL, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr));
+        BRTM_TRY(inject_data(c, it, jt, lo, snap, FMT_F32, nullptr, img));
+      } else {
+        BRTM_TRY(step(c, FORM_VL, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr, nullptr));
+        BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
+      }
+      std::swap(lc, lo);
+    }
+  }
+  return BRTM_OK;
+}
+
+int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out) {
+  BRTM_TRY(c->partials.alloc(sizeof(double) * kDotBlocks));
+  BRTM_TRY(c->dotv.alloc(sizeof(double)));
+  BRTM_HIP(launch_dot(a, b, n, c->partials.as<double>(), c->dotv.as<double>(), c->stream));
+  BRTM_HIP(hipMemcpyAsync(out, c->dotv.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+}  // namespace brtm
+
+using namespace brtm;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *brtm_last_error(void) { return g_err.c_str(); }
+const char *brtm_version(void) { return "sep_code_amd born_rtm 0.1 (gfx950)"; }
+
+int brtm_create(int device, brtm_ctx **out) {
+  BRTM_REQUIRE(out, BRTM_EINVAL, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  BRTM_HIP(hipGetDeviceCount(&ndev));
+  BRTM_REQUIRE(device >= 0 && device < ndev, BRTM_EINVAL, "no such device");
+  BRTM_HIP(hipSetDevice(device));
+  brtm_ctx *c = new brtm_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    set_error("stream/event creation failed");
+    return BRTM_EHIP;
+  }
+  *out = c;
+  return BRTM_OK;
+}
+
+int brtm_destroy(brtm_ctx *c) {
+  if (!c) return BRTM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &s : c->stores) {
+    s.data.release();
+    s.scale.release();
+  }
+  for (DevBuf *b : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs,
+                    &c->traces})
+    b->release();
+  for (hipEvent_t e : c->ktime_ev) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BRTM_OK;
+}
+
+int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, float bc_b, int n1,
+                      int n2, int n3) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(n1 >= 9 && n2 >= 9 && n3 >= 9, BRTM_EINVAL, "each axis needs at least 9 cells");
+  BRTM_REQUIRE((double)n1 * n2 * n3 < 2147483647.0, BRTM_EINVAL,
+               "grid exceeds 2^31 cells (32-bit cell indexing)");
+  BRTM_REQUIRE(d1 > 0 && d2 > 0 && d3 > 0, BRTM_EINVAL, "grid spacing must be positive");
+  BRTM_REQUIRE(bc_a >= 0 && bc_a <= (float)kWMax && bc_b >= 0, BRTM_EINVAL,
+               "sponge parameters out of range (0 <= bc_a <= 256, bc_b >= 0)");
+  BRTM_HIP(hipSetDevice(c->device));
+  c->n1 = n1;
+  c->n2 = n2;
+  c->n3 = n3;
+  c->ncell = (long)n1 * n2 * n3;
+  c->d[0] = d1;
+  c->d[1] = d2;
+  c->d[2] = d3;
+  c->bc_a = bc_a;
+  c->bc_b = bc_b;
+  // Born/wave_fkernel.3d8o.cu:26-27 evaluated in double.
+  const double C[3] = {8.0 / (5.0 * (double)d1 * d1), 8.0 / (5.0 * (double)d2 * d2),
+                       8.0 / (5.0 * (double)d3 * d3)};
+  const double f[4] = {1.0, -1.0 / 8.0, 1.0 / 63.0, -1.0 / 896.0};
+  c->coeffs[0] = (float)(-1025.0 / 576.0 * (C[0] + C[1] + C[2]));
+  for (int k = 0; k < 4; ++k)
+    for (int a = 0; a < 3; ++a) c->coeffs[1 + 3 * k + a] = (float)(C[a] * f[k]);
+  // Sponge weights in FP64 (Born/wave_fkernel.3d8o.cu:201-202).
+  c->nw = (int)std::ceil((double)bc_a);
+  c->wtab.assign(std::max(c->nw, 1), 1.f);
+  for (int e = 0; e < c->nw; ++e) {
+    double w = std::exp(-(double)bc_b * ((double)bc_a - e));
+    c->wtab[e] = (float)(w < 1.0 ? w : 1.0);
+  }
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  BRTM_TRY(c->d_wtab.alloc(sizeof(float) * c->wtab.size()));
+  BRTM_HIP(hipMemcpy(c->d_wtab.ptr, c->wtab.data(), sizeof(float) * c->wtab.size(),
+                     hipMemcpyHostToDevice));
+  for (DevBuf *b : {&c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl}) {
+    b->release();
+    BRTM_TRY(b->alloc(fbytes));
+    BRTM_HIP(hipMemsetAsync(b->ptr, 0, fbytes, c->stream));
+  }
+  c->seg.release();
+  c->seg_slots = 0;
+  for (auto &s : c->stores) {
+    s.data.release();
+    s.scale.release();
+    s.kind = BRTM_STORE_NONE;
+  }
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  c->have_space = true;
+  c->have_vel = false;
+  if (c->sinc[0].empty()) c->sinc[0] = {0, 0, 0, 1, 0, 0, 0, 0};
+  if (c->sinc[1].empty()) c->sinc[1] = {0, 0, 0, 1, 0, 0, 0, 0};
+  return BRTM_OK;
+}
+
+int brtm_set_velocity(brtm_ctx *c, const float *V) {
+  BRTM_REQUIRE(c && V, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  // Leapfrog limit: max V * lambda_max(-L8) <= 4, lambda_max(-L8) = sum_a
+  // C_a (1025/576 + 2 (1 + 1/8 + 1/63 + 1/896)) (symbol of L8 at pi).
+  double vmax = 0;
+  for (long i = 0; i < c->ncell; ++i) {
+    const double v = V[i];
+    BRTM_REQUIRE(std::isfinite(v) && v >= 0, BRTM_EINVAL, "V must be finite and >= 0");
+    vmax = std::max(vmax, v);
+  }
+  const double S = 1025.0 / 576.0 + 2.0 * (1.0 + 1.0 / 8 + 1.0 / 63 + 1.0 / 896);
+  double lam = 0;
+  for (int a = 0; a < 3; ++a) lam += 8.0 / (5.0 * (double)c->d[a] * c->d[a]) * S;
+  BRTM_REQUIRE(vmax * lam <= 4.0, BRTM_EUNSTABLE,
+               "velocity violates the 3-D 8th-order leapfrog stability limit "
+               "(max v dt / d above 0.4529 for equal spacing)");
+  BRTM_HIP(hipSetDevice(c->device));
+  BRTM_HIP(hipMemcpy(c->V.ptr, V, sizeof(float) * (size_t)c->ncell, hipMemcpyHostToDevice));
+  c->have_vel = true;
+  return BRTM_OK;
+}
+
+int brtm_set_sinc_table(brtm_ctx *c, int which, int nsinc, int ntab, const float *table) {
+  BRTM_REQUIRE(c && table, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(which == 0 || which == 1, BRTM_EINVAL, "which must be 0 (source) or 1 (data)");
+  BRTM_REQUIRE(nsinc == 8, BRTM_EINVAL, "only 8-tap interpolation tables are supported");
+  BRTM_REQUIRE(ntab >= 1, BRTM_EINVAL, "ntab must be >= 1");
+  c->sinc[which].assign(table, table + 8 * ntab);
+  return BRTM_OK;
+}
+
+static bool in_update_set(const brtm_ctx *c, long loc) {
+  if (loc < 0 || loc >= c->ncell) return false;
+  const int i1 = (int)(loc % c->n1), i2 = (int)((loc / c->n1) % c->n2),
+            i3 = (int)(loc / ((long)c->n1 * c->n2));
+  return i1 >= 4 && i1 < c->n1 - 4 && i2 >= 4 && i2 < c->n2 - 4 && i3 >= 4 && i3 < c->n3 - 4;
+}
+
+int brtm_set_source(brtm_ctx *c, int npts, int ntr, const int *locs, const float *vals) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_REQUIRE(npts >= 0 && ntr >= 1 && (npts == 0 || (locs && vals)), BRTM_EINVAL,
+               "bad source arguments");
+  for (int j = 0; j < npts; ++j)
+    BRTM_REQUIRE(in_update_set(c, locs[j]), BRTM_EINVAL,
+                 "source point " + std::to_string(j) + " outside the update set");
+  BRTM_HIP(hipSetDevice(c->device));
+  c->nsrc = npts;
+  c->ntr_s = ntr;
+  if (npts == 0) return BRTM_OK;
+  BRTM_TRY(c->slocs.alloc(sizeof(int) * npts));
+  BRTM_TRY(c->strc.alloc(sizeof(float) * (size_t)npts * ntr));
+  BRTM_HIP(hipMemcpy(c->slocs.ptr, locs, sizeof(int) * npts, hipMemcpyHostToDevice));
+  BRTM_HIP(hipMemcpy(c->strc.ptr, vals, sizeof(float) * (size_t)npts * ntr,
+                     hipMemcpyHostToDevice));
+  return BRTM_OK;
+}
+
+int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, const float *vals) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_REQUIRE(nx >= 0 && ny >= 0 && ntr >= 8 && (nx * ny == 0 || locs), BRTM_EINVAL,
+               "bad receiver arguments");
+  BRTM_HIP(hipSetDevice(c->device));
+  const int nrec = nx * ny;
+  c->nrx = nx;
+  c->nry = ny;
+  c->nrec = nrec;
+  c->ntr_d = ntr;
+  if (nrec == 0) return BRTM_OK;
+  // Receivers outside the update set read a field that is always zero and
+  // would write into cells no step updates: mark them inactive.
+  c->h_rlocs.assign(locs, locs + nrec);
+  for (auto &l : c->h_rlocs)
+    if (!in_update_set(c, l)) l = -1;
+  BRTM_TRY(c->rlocs.alloc(sizeof(int) * nrec));
+  BRTM_HIP(hipMemcpy(c->rlocs.ptr, c->h_rlocs.data(), sizeof(int) * nrec, hipMemcpyHostToDevice));
+  const size_t tb = sizeof(float) * (size_t)nrec * ntr;
+  BRTM_TRY(c->traces.alloc(tb));
+  if (!vals) {
+    BRTM_HIP(hipMemsetAsync(c->traces.ptr, 0, tb, c->stream));
+  } else {
+    DevBuf tmp;
+    BRTM_TRY(tmp.alloc(tb));
+    BRTM_HIP(hipMemcpy(tmp.ptr, vals, tb, hipMemcpyHostToDevice));
+    BRTM_HIP(launch_transpose(tmp.as<float>(), c->traces.as<float>(), nrec, ntr, c->stream));
+    BRTM_HIP(hipStreamSynchronize(c->stream));
+    tmp.release();
+  }
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int brtm_get_receivers(brtm_ctx *c, float *vals) {
+  BRTM_REQUIRE(c && vals, BRTM_EINVAL, "NULL argument");
+  if (c->nrec == 0) return BRTM_OK;
+  BRTM_HIP(hipSetDevice(c->device));
+  const size_t tb = sizeof(float) * (size_t)c->nrec * c->ntr_d;
+  DevBuf tmp;
+  BRTM_TRY(tmp.alloc(tb));
+  BRTM_HIP(launch_transpose(c->traces.as<float>(), tmp.as<float>(), c->ntr_d, c->nrec, c->stream));
+  BRTM_HIP(hipMemcpyAsync(vals, tmp.ptr, tb, hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  tmp.release();
+  return BRTM_OK;
+}
+
+int brtm_set_model(brtm_ctx *c, const float *m) {
+  BRTM_REQUIRE(c && m, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_HIP(hipSetDevice(c->device));
+  BRTM_HIP(hipMemcpy(c->mdl.ptr, m, sizeof(float) * (size_t)c->ncell, hipMemcpyHostToDevice));
+  return BRTM_OK;
+}
+
+int brtm_get_image(brtm_ctx *c, float *img) {
+  BRTM_REQUIRE(c && img, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_HIP(hipSetDevice(c->device));
+  BRTM_HIP(hipMemcpyAsync(img, c->img.ptr, sizeof(float) * (size_t)c->ncell,
+                          hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int brtm_zero_image(brtm_ctx *c) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
+  BRTM_HIP(hipMemsetAsync(c->img.ptr, 0, sizeof(float) * (size_t)c->ncell, c->stream));
+  return BRTM_OK;
+}
+
+int brtm_select_shot(brtm_ctx *c, int ishot) {
+  BRTM_REQUIRE(c && ishot >= 0, BRTM_EINVAL, "bad shot index");
+  c->shot = ishot;
+  cur_store(c);
+  return BRTM_OK;
+}
+
+int brtm_release_store(brtm_ctx *c, int ishot) {
+  BRTM_REQUIRE(c && ishot >= 0, BRTM_EINVAL, "bad shot index");
+  if (ishot < (int)c->stores.size()) {
+    c->stores[ishot].data.release();
+    c->stores[ishot].scale.release();
+    c->stores[ishot].kind = BRTM_STORE_NONE;
+  }
+  return BRTM_OK;
+}
+
+int brtm_source_prop(brtm_ctx *c, int jt, int nt, int jimg, int store, int ckpt, float *p0,
+                     float *p1) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_HIP(hipSetDevice(c->device));
+  return source_prop(c, jt, nt, jimg, store, ckpt, p0, p1);
+}
+
+int brtm_rtm_forward(brtm_ctx *c, int jt, int nt, int jimg, int accumulate) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_HIP(hipSetDevice(c->device));
+  return born_forward(c, jt, nt, jimg, accumulate != 0);
+}
+
+int brtm_rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_HIP(hipSetDevice(c->device));
+  return rtm_adjoint(c, jt, nt, jimg);
+}
+
+int brtm_set_imaging_scale(brtm_ctx *c, float cimg) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  c->cimg = cimg;
+  return BRTM_OK;
+}
+
+int brtm_get_snapshot(brtm_ctx *c, int k, float *out) {
+  BRTM_REQUIRE(c && out, BRTM_EINVAL, "NULL argument");
+  Store &S = cur_store(c);
+  const int fmt = fmt_of_store(S.kind);
+  BRTM_REQUIRE(fmt != FMT_NONE, BRTM_ESTATE, "selected shot has no snapshot store");
+  BRTM_REQUIRE(k >= 0 && k < S.count, BRTM_EINVAL, "snapshot index out of range");
+  BRTM_HIP(hipSetDevice(c->device));
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  if (fmt == FMT_F32) {
+    BRTM_HIP(hipMemcpyAsync(out, S.data.as<char>() + fbytes * (size_t)k, fbytes,
+                            hipMemcpyDeviceToHost, c->stream));
+  } else {
+    DevBuf tmp;
+    BRTM_TRY(tmp.alloc(fbytes));
+    BRTM_HIP(launch_decode(fmt, S.data.as<char>() + 2 * (size_t)c->ncell * k,
+                           S.scale.as<float>() + (size_t)c->n3 * k, tmp.as<float>(),
+                           c->n1 * c->n2, c->n3, c->stream));
+    BRTM_HIP(hipMemcpyAsync(out, tmp.ptr, fbytes, hipMemcpyDeviceToHost, c->stream));
+    BRTM_HIP(hipStreamSynchronize(c->stream));
+  }
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int brtm_propagate(brtm_ctx *c, int form, int nsteps, float *p_cur, float *p_old) {
+  BRTM_REQUIRE(c && p_cur && p_old, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(form == 0 || form == 1, BRTM_EINVAL, "form must be 0 (VL) or 1 (LV)");
+  BRTM_REQUIRE(c->have_space && c->have_vel, BRTM_ESTATE, "space/velocity not set");
+  BRTM_HIP(hipSetDevice(c->device));
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  float *cur = c->fb.as<float>(), *old = c->fa.as<float>();
+  BRTM_HIP(hipMemcpyAsync(cur, p_cur, fbytes, hipMemcpyHostToDevice, c->stream));
+  BRTM_HIP(hipMemcpyAsync(old, p_old, fbytes, hipMemcpyHostToDevice, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    BRTM_TRY(step(c, form, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr, nullptr));
+    std::swap(cur, old);
+  }
+  BRTM_HIP(hipMemcpyAsync(p_cur, cur, fbytes, hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipMemcpyAsync(p_old, old, fbytes, hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int brtm_time_step(brtm_ctx *c, int form, int fmt, int nrep, double *avg_ms) {
+  BRTM_REQUIRE(c && avg_ms && nrep >= 1, BRTM_EINVAL, "bad arguments");
+  BRTM_REQUIRE(form == 0 || form == 1, BRTM_EINVAL, "form must be 0 or 1");
+  BRTM_REQUIRE(fmt >= FMT_NONE && fmt <= FMT_F16, BRTM_EINVAL, "fmt must be 0..3");
+  BRTM_REQUIRE(c->have_space && c->have_vel, BRTM_ESTATE, "space/velocity not set");
+  BRTM_HIP(hipSetDevice(c->device));
+  DevBuf sc;
+  BRTM_TRY(sc.alloc(sizeof(float) * c->n3));
+  BRTM_HIP(launch_fill(sc.as<float>(), 1.f, c->n3, c->stream));
+  float *cur = c->fb.as<float>(), *old = c->fa.as<float>();
+  const void *snap = c->fd.ptr;  // any resident field serves as the snapshot
+  float *img = form == 0 && fmt ? c->img.as<float>() : nullptr;
+  const float *m = form == 1 && fmt ? c->mdl.as<float>() : nullptr;
+  // warm-up launch, then the timed run
+  BRTM_TRY(step(c, form, fmt, cur, old, old, snap, sc.as<float>(), img, m));
+  BRTM_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < nrep; ++s) {
+    BRTM_TRY(step(c, form, fmt, cur, old, old, snap, sc.as<float>(), img, m));
+    std::swap(cur, old);
+  }
+  BRTM_HIP(hipEventRecord(c->ev1, c->stream));
+  BRTM_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  BRTM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *avg_ms = ms / nrep;
+  sc.release();
+  return BRTM_OK;
+}
+
+int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(ty_tile == 8 || ty_tile == 16 || ty_tile == 32, BRTM_EINVAL,
+               "ty_tile must be 8, 16 or 32");
+  BRTM_REQUIRE(target_blocks >= 1, BRTM_EINVAL, "target_blocks must be >= 1");
+  c->ty_tile = ty_tile;
+  c->target_blocks = target_blocks;
+  return BRTM_OK;
+}
+
+int brtm_kernel_timing(brtm_ctx *c, int enable, int form, int fmt) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  c->ktime_on = enable != 0;
+  c->ktime_form = form;
+  c->ktime_fmt = fmt;
+  c->ktime_used = 0;
+  return BRTM_OK;
+}
+
+int brtm_kernel_timing_result(brtm_ctx *c, double *total_ms, int *count) {
+  BRTM_REQUIRE(c && total_ms && count, BRTM_EINVAL, "NULL argument");
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  double t = 0;
+  for (size_t k = 0; k + 1 < c->ktime_used; k += 2) {
+    float ms = 0;
+    BRTM_HIP(hipEventElapsedTime(&ms, c->ktime_ev[k], c->ktime_ev[k + 1]));
+    t += ms;
+  }
+  *total_ms = t;
+  *count = (int)(c->ktime_used / 2);
+  return BRTM_OK;
+}
+
+int brtm_sync(brtm_ctx *c) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_HIP(hipStreamSynchronize(c->stream));
+  return BRTM_OK;
+}
+
+int brtm_device_bytes(brtm_ctx *c, double *bytes) {
+  BRTM_REQUIRE(c && bytes, BRTM_EINVAL, "NULL argument");
+  double b = 0;
+  for (DevBuf *x : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs,
+                    &c->traces})
+    b += (double)x->bytes;
+  for (auto &s : c->stores) b += (double)s.data.bytes + (double)s.scale.bytes;
+  *bytes = b;
+  return BRTM_OK;
+}
+
+}  // extern "C"

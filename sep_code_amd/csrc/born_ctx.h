@@ -1,0 +1,115 @@
+// Internal context of the Born RTM library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "born_kernels.h"
+
+struct brtm_ctx;
+
+namespace brtm {
+
+void set_error(const std::string &msg);
+
+#define BRTM_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::brtm::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return BRTM_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+#define BRTM_REQUIRE(cond, code, msg)   \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::brtm::set_error(msg);           \
+      return (code);                    \
+    }                                   \
+  } while (0)
+
+// Device allocation owned by a context.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  int alloc(size_t n);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// One shot's source wavefield.
+struct Store {
+  int kind = 0;    // BRTM_STORE_*
+  int nt = 0, jt = 1, jimg = 1, ckpt = 0, count = 0;
+  DevBuf data;     // snapshots [count][ncell] or checkpoints [count][2][ncell]
+  DevBuf scale;    // [count][n3] slice scales (16-bit formats)
+};
+
+}  // namespace brtm
+
+struct brtm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool have_space = false, have_vel = false;
+  int n1 = 0, n2 = 0, n3 = 0;
+  long ncell = 0;
+  float d[3] = {0, 0, 0};
+  float bc_a = 0, bc_b = 0;
+  float coeffs[13];
+  int nw = 0;
+  std::vector<float> wtab;
+  float cimg = 1e-6f;
+  int ty_tile = 16, target_blocks = 2048;
+
+  brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
+  int seg_slots = 0;
+  brtm::DevBuf partials, dotv;
+
+  // interpolation tables [jt][8]
+  std::vector<float> sinc[2];
+
+  // source geometry and traces
+  int nsrc = 0, ntr_s = 0;
+  brtm::DevBuf slocs, strc;
+  // receivers
+  int nrx = 0, nry = 0, nrec = 0, ntr_d = 0;
+  brtm::DevBuf rlocs, traces;  // traces [ntr_d][nrec]
+  std::vector<int> h_rlocs;
+
+  std::vector<brtm::Store> stores;
+  int shot = 0;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // Per-launch timing of one (form, fmt) stencil variant: an event pair is
+  // recorded around every such launch while enabled.
+  bool ktime_on = false;
+  int ktime_form = 0, ktime_fmt = 0;
+  std::vector<hipEvent_t> ktime_ev;  // 2 per launch
+  size_t ktime_used = 0;
+};
+
+namespace brtm {
+
+// Internal propagation entry points (used by the C ABI and the operator).
+int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int store, int ckpt, float *p0,
+                float *p1);
+int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate);
+int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg);
+int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out);
+Store &cur_store(brtm_ctx *c);
+int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float *pn,
+         const void *snap, const float *sscale, float *img_or_null, const float *m_or_null);
+
+}  // namespace brtm

@@ -1,0 +1,97 @@
+// Device-side parameter blocks and launchers for the Born RTM kernels
+// (gfx950).  Host code in born_context.hip includes this header.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace brtm {
+
+// Storage format of a source-wavefield snapshot.
+enum SnapFmt { FMT_NONE = 0, FMT_F32 = 1, FMT_BF16 = 2, FMT_F16 = 3 };
+// Step form: VL = V.L(p) (Born/wave_fkernel.3d8o.cu:31), LV = L(V.p) (:109).
+enum StepForm { FORM_VL = 0, FORM_LV = 1 };
+
+constexpr int kWMax = 256;  // longest sponge table the kernels accept
+
+// One time step of the fused stencil:
+//   VL: pn = w.(V.L p + 2p) - w^2.po          [+ img += cimg.s.pn]
+//   LV: pn = L(V.w.p) + 2w.p - w^2.po          [+ pn  += cimg.s.m ]
+// s is the source snapshot (format fmt, per-slice scale sscale for 16-bit).
+struct StepParams {
+  int n1, n2, n3, nw;
+  int ntx, nty, ntz, zc;
+  float c[13];
+  const float *p;
+  const float *po;
+  float *pn;            // may alias po
+  const float *V;
+  const float *wtab;    // nw sponge weights
+  float *img;           // VL: image accumulator
+  const void *snap;     // source snapshot
+  const float *sscale;  // [n3] slice scales (16-bit formats), else null
+  const float *m;       // LV: Born model
+  float cimg;
+};
+
+hipError_t launch_step(int form, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// Tile / chunk geometry chosen for a grid (fills ntx..zc of P).
+void plan_step(StepParams &P, int ty_tile, int target_blocks);
+
+// p[loc_j] += scale * sum_k sinc8[k] tr(j, id+k), tr(j,t) = tr[j*sp + t*st].
+// When img != null: img[loc_j] += cimg * s[loc_j] * (added amount).
+struct InjectParams {
+  int npts;
+  const int *locs;
+  const float *tr;
+  long sp, st;
+  int id;
+  float sinc8[8];
+  float scale;
+  float *p;
+  float *img;
+  const void *snap;
+  const float *sscale;
+  int fmt;
+  int n12;
+  float cimg;
+};
+hipError_t launch_inject(const InjectParams &P, hipStream_t st);
+
+// tr(j, id+k) += sinc8[k] * p[loc_j]  (device trace layout [t][rec]).
+hipError_t launch_extract(int npts, const int *locs, float *tr, long st_t, int id,
+                          const float sinc8[8], const float *p, hipStream_t st);
+
+// Snapshot encoding: per-slice max |s| then quantise to fmt.
+hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int n12, int n3,
+                         hipStream_t st);
+// Decode one snapshot to FP32 (tests, diagnostics).
+hipError_t launch_decode(int fmt, const void *src, const float *sscale, float *dst, int n12,
+                         int n3, hipStream_t st);
+
+// FP64 reductions / vector kernels.
+constexpr int kDotBlocks = 1024;
+hipError_t launch_dot(const float *a, const float *b, long n, double *partials,
+                      double *out, hipStream_t st);
+hipError_t launch_axpby(float *y, const float *x, double a, double b, long n,
+                        hipStream_t st);  // y = a*y + b*x
+hipError_t launch_mul(float *y, const float *x, long n, hipStream_t st);  // y *= x
+hipError_t launch_fill(float *y, float v, long n, hipStream_t st);
+
+// Volume operators (compute grid [n3][n2][n1]).
+hipError_t launch_dz(int adj, const float *in, float *out, int n12, int n3, int add,
+                     hipStream_t st);
+// Embed a model-grid volume into the compute grid at offset (o1,o2,o3)
+// (adj = 0), or accumulate the compute-grid window back (adj = 1).
+hipError_t launch_embed(int adj, const float *model, float *comp, int m1, int m2, int m3,
+                        int n1, int n2, int n3, int o1, int o2, int o3, hipStream_t st);
+// Traces: device [ntb][nrec] <-> data [nrec][nt] with window (drop 4) and
+// time derivative d[t] = (w[t-1] - w[t]) / dt.
+hipError_t launch_trace_to_data(const float *tr, float *data, int nrec, int ntb, int nt,
+                                float inv_dt, int add, hipStream_t st);
+hipError_t launch_data_to_trace(const float *data, float *tr, int nrec, int ntb, int nt,
+                                float inv_dt, hipStream_t st);
+// Plain transpose [rows][cols] -> [cols][rows].
+hipError_t launch_transpose(const float *in, float *out, long rows, long cols,
+                            hipStream_t st);
+
+}  // namespace brtm

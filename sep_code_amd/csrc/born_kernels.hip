@@ -1,0 +1,641 @@
+// Born acoustic RTM kernels for gfx950 (MI355X).
+//
+// The stencil is a 2.5D sweep: a workgroup owns a 64 x TY column tile of the
+// x-y plane (x fastest, tiles aligned to 64 floats so each wave row is one
+// 256-byte coalesced access) and streams a chunk of the slow axis z.  Each
+// thread keeps a 9-deep register queue of its column's z taps; the x-y plane
+// of the current slice is staged in LDS with a 4-cell halo (double-buffered,
+// one barrier per slice).  The next slice's loads are issued before the
+// barrier so HBM latency overlaps the current slice's arithmetic.  Workgroup
+// ids are remapped so each XCD (dispatched round-robin) owns a contiguous
+// run of tiles: neighbouring tiles, whose halos overlap, share that XCD's L2.
+//
+// Reference kernels restated (semantics, not code):
+//   wave_kernel      Born/wave_fkernel.3d8o.cu:31-106   (FORM_VL)
+//   wave_kernel_adj  Born/wave_fkernel.3d8o.cu:109-181  (FORM_LV)
+//   damp_kernel      Born/wave_fkernel.3d8o.cu:183-210  (fused, table weights)
+//   img_kernel       Born/gpu_funcs_3d.cu:111-120       (fused into FORM_VL)
+//   img_add_kernel   Born/gpu_funcs_3d.cu:122-131       (fused into FORM_LV)
+//   new_src_inject / new_data_inject  Born/gpu_funcs_3d.cu:64-93  (k_inject)
+//   new_data_extract Born/gpu_funcs_3d.cu:95-109        (k_extract)
+#include "born_kernels.h"
+
+#include <hip/hip_fp16.h>
+
+namespace brtm {
+
+namespace {
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <int FMT>
+__device__ __forceinline__ float snap_at(const void *s, const float *sscale, int i, int z) {
+  if constexpr (FMT == FMT_F32) {
+    return static_cast<const float *>(s)[i];
+  } else if constexpr (FMT == FMT_BF16) {
+    return bf16_to_f32(static_cast<const uint16_t *>(s)[i]) * sscale[z];
+  } else if constexpr (FMT == FMT_F16) {
+    return __half2float(static_cast<const __half *>(s)[i]) * sscale[z];
+  } else {
+    return 0.f;
+  }
+}
+
+__device__ __forceinline__ float snap_dyn(int fmt, const void *s, const float *sscale, int i,
+                                          int z) {
+  switch (fmt) {
+    case FMT_F32: return snap_at<FMT_F32>(s, sscale, i, z);
+    case FMT_BF16: return snap_at<FMT_BF16>(s, sscale, i, z);
+    case FMT_F16: return snap_at<FMT_F16>(s, sscale, i, z);
+    default: return 0.f;
+  }
+}
+
+// Edge distance of Born/wave_fkernel.3d8o.cu:194 along one axis.
+__device__ __forceinline__ int edist(int i, int n) { return min(i - 4, n - 4 - i); }
+
+// ---------------------------------------------------------------------------
+// The fused stencil step.
+// ---------------------------------------------------------------------------
+template <int FORM, int TY, int RPT, int FMT>
+__global__ void __launch_bounds__(64 * (TY / RPT))
+k_step(const StepParams P) {
+  constexpr int NWV = TY / RPT;  // waves per workgroup
+  constexpr int HY = 8 / NWV;    // y-halo rows loaded per wave
+  static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
+  constexpr int LW = 72;         // LDS row: 4 + 64 + 4
+  __shared__ float plane[2][TY + 8][LW];
+  __shared__ float wsh[kWMax];
+
+  const int lane = threadIdx.x;
+  const int wy = threadIdx.y;
+  const int n1 = P.n1, n2 = P.n2, n3 = P.n3, nw = P.nw;
+  const int s3 = n1 * n2;
+  for (int e = wy * 64 + lane; e < nw; e += 64 * NWV) wsh[e] = P.wtab[e];
+
+  // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
+  // contiguous range of tiles (x fastest, then y, then z chunk).
+  const int nb = P.ntx * P.nty * P.ntz;
+  const int bid = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = bid & 7, jj = bid >> 3;
+  const int t = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const int tx = t % P.ntx;
+  const int ty = (t / P.ntx) % P.nty;
+  const int tz = t / (P.ntx * P.nty);
+
+  const int x = tx * 64 + lane;
+  const int y0 = ty * TY;
+  const int yb = y0 + wy * RPT;
+  const int z0 = 4 + tz * P.zc;
+  const int z1 = min(z0 + P.zc, n3 - 4);
+  const bool xmem = x < n1;
+  const bool xin = (x >= 4) && (x < n1 - 4);
+  const int ex = edist(x, n1);
+
+  // Staged value at (xx, y, z): p for VL; V.w.p for LV.
+  auto stage = [&](int xx, int y, int z, bool ok, float &raw) -> float {
+    if (!ok) { raw = 0.f; return 0.f; }
+    const int i = z * s3 + y * n1 + xx;
+    const float v = P.p[i];
+    raw = v;
+    if constexpr (FORM == FORM_LV) {
+      const int e = min(min(edist(xx, n1), edist(y, n2)), edist(z, n3));
+      // p is zero outside the update set, so any weight works there.
+      const float w = (e >= 0 && e < nw) ? P.wtab[e] : 1.f;
+      return P.V[i] * w * v;
+    } else {
+      return v;
+    }
+  };
+
+  // ---- prologue: register queue for z0-4 .. z0+4 -------------------------
+  float q[RPT][9];
+  float rq[RPT][5];  // LV: raw p at z .. z+4
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = yb + r;
+    const bool ok = xmem && y < n2;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      float raw;
+      q[r][k] = stage(x, y, z0 - 4 + k, ok, raw);
+      if constexpr (FORM == FORM_LV) {
+        if (k >= 4) rq[r][k - 4] = raw;
+      }
+    }
+  }
+
+  // Halo coordinates (fixed per thread).
+  int hyy[HY], hyrow[HY];
+#pragma unroll
+  for (int j = 0; j < HY; ++j) {
+    const int h = wy * HY + j;
+    hyy[j] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+    hyrow[j] = h < 4 ? h : TY + h;
+  }
+  const bool hxl = lane < 8;
+  const int hxx = lane < 4 ? tx * 64 - 4 + lane : tx * 64 + 60 + lane;
+  const int hxcol = lane < 4 ? lane : 64 + lane;
+  const bool hxok = hxl && hxx >= 0 && hxx < n1;
+
+  float hy[HY], hx[RPT];
+  float cpo[RPT], cV[RPT], cS[RPT], cX[RPT];  // cX: img (VL) or m (LV)
+
+  auto load_halo = [&](int z, float *ohy, float *ohx) {
+#pragma unroll
+    for (int j = 0; j < HY; ++j) {
+      float raw;
+      ohy[j] = stage(x, hyy[j], z, xmem && hyy[j] >= 0 && hyy[j] < n2, raw);
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      float raw;
+      ohx[r] = stage(hxx, yb + r, z, hxok && (yb + r) < n2, raw);
+    }
+  };
+  auto load_center = [&](int z, float *opo, float *oV, float *oS, float *oX) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const bool ok = xmem && y < n2;
+      const int i = z * s3 + y * n1 + x;
+      opo[r] = ok ? P.po[i] : 0.f;
+      if constexpr (FORM == FORM_VL) oV[r] = ok ? P.V[i] : 0.f;
+      if constexpr (FMT != FMT_NONE) {
+        oS[r] = ok ? snap_at<FMT>(P.snap, P.sscale, i, z) : 0.f;
+        oX[r] = ok ? (FORM == FORM_VL ? P.img[i] : P.m[i]) : 0.f;
+      }
+    }
+  };
+
+  load_halo(z0, hy, hx);
+  load_center(z0, cpo, cV, cS, cX);
+
+  const float c0 = P.c[0];
+  for (int z = z0; z < z1; ++z) {
+    float(*pl)[LW] = plane[z & 1];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) pl[4 + wy * RPT + r][lane + 4] = q[r][4];
+#pragma unroll
+    for (int j = 0; j < HY; ++j) pl[hyrow[j]][lane + 4] = hy[j];
+    if (hxl) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) pl[4 + wy * RPT + r][hxcol] = hx[r];
+    }
+
+    // Prefetch the next slice before the barrier.
+    const int zn = z + 1;
+    float nq[RPT], nraw[RPT], nhy[HY], nhx[RPT];
+    float npo[RPT], nV[RPT], nS[RPT], nX[RPT];
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int y = yb + r;
+        nq[r] = stage(x, y, zn + 4, xmem && y < n2, nraw[r]);
+      }
+      load_halo(zn, nhy, nhx);
+      load_center(zn, npo, nV, nS, nX);
+    }
+    __syncthreads();
+
+    const int ez = edist(z, n3);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const int row = 4 + wy * RPT + r;
+      const int col = lane + 4;
+      const float cc = q[r][4];
+      float acc = c0 * cc;
+#pragma unroll
+      for (int k = 1; k <= 4; ++k) {
+        acc = fmaf(P.c[1 + 3 * (k - 1)], pl[row][col - k] + pl[row][col + k], acc);
+        acc = fmaf(P.c[2 + 3 * (k - 1)], pl[row - k][col] + pl[row + k][col], acc);
+        acc = fmaf(P.c[3 + 3 * (k - 1)], q[r][4 - k] + q[r][4 + k], acc);
+      }
+      const bool in = xin && y >= 4 && y < n2 - 4;
+      if (in) {
+        const int e = min(min(ex, edist(y, n2)), ez);
+        const float w = e < nw ? wsh[e] : 1.f;
+        const int i = z * s3 + y * n1 + x;
+        float pn;
+        if constexpr (FORM == FORM_VL) {
+          pn = w * fmaf(cV[r], acc, 2.f * cc) - (w * w) * cpo[r];
+          P.pn[i] = pn;
+          if constexpr (FMT != FMT_NONE) P.img[i] = fmaf(P.cimg * cS[r], pn, cX[r]);
+        } else {
+          pn = acc + 2.f * w * rq[r][0] - (w * w) * cpo[r];
+          if constexpr (FMT != FMT_NONE) pn = fmaf(P.cimg * cS[r], cX[r], pn);
+          P.pn[i] = pn;
+        }
+      }
+    }
+
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[r][k] = q[r][k + 1];
+        q[r][8] = nq[r];
+        if constexpr (FORM == FORM_LV) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) rq[r][k] = rq[r][k + 1];
+          rq[r][4] = nraw[r];
+        }
+        cpo[r] = npo[r];
+        cV[r] = nV[r];
+        cS[r] = nS[r];
+        cX[r] = nX[r];
+        hx[r] = nhx[r];
+      }
+#pragma unroll
+      for (int j = 0; j < HY; ++j) hy[j] = nhy[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Injection / extraction (one thread per point).
+// ---------------------------------------------------------------------------
+__global__ void k_inject(const InjectParams P) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P.npts) return;
+  const int loc = P.locs[j];
+  if (loc < 0) return;
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (P.sinc8[k] != 0.f) acc = fmaf(P.sinc8[k], P.tr[j * P.sp + (long)(P.id + k) * P.st], acc);
+  }
+  const float add = P.scale * acc;
+  atomicAdd(P.p + loc, add);
+  if (P.img) {
+    const float s = snap_dyn(P.fmt, P.snap, P.sscale, loc, loc / P.n12);
+    atomicAdd(P.img + loc, P.cimg * s * add);
+  }
+}
+
+__global__ void k_extract(int npts, const int *__restrict__ locs, float *tr, long st_t, int id,
+                          float s0, float s1, float s2, float s3, float s4, float s5, float s6,
+                          float s7, const float *__restrict__ p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  const int loc = locs[j];
+  if (loc < 0) return;
+  const float v = p[loc];
+  const float s[8] = {s0, s1, s2, s3, s4, s5, s6, s7};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (s[k] != 0.f) tr[(long)(id + k) * st_t + j] += v * s[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot codec: scale = max |s| over each z slice; q = s / scale.
+// ---------------------------------------------------------------------------
+__global__ void k_slice_absmax(const float *__restrict__ s, unsigned int *smax, int n12) {
+  const int z = blockIdx.y;
+  const float *sl = s + (long)z * n12;
+  float m = 0.f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(sl[i]));
+  __shared__ float red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  // Non-negative floats order like their bit patterns.
+  if (threadIdx.x == 0) atomicMax(smax + z, __float_as_uint(red[0]));
+}
+
+template <int FMT>
+__global__ void k_encode(const float *__restrict__ s, void *dst, const float *__restrict__ sscale,
+                         int n12) {
+  const int z = blockIdx.y;
+  const float sc = sscale[z];
+  const float inv = sc > 0.f ? 1.f / sc : 0.f;
+  const long base = (long)z * n12;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += gridDim.x * blockDim.x) {
+    const float v = s[base + i] * inv;
+    if constexpr (FMT == FMT_BF16)
+      static_cast<uint16_t *>(dst)[base + i] = f32_to_bf16_rne(v);
+    else
+      static_cast<__half *>(dst)[base + i] = __float2half_rn(v);
+  }
+}
+
+__global__ void k_decode(int fmt, const void *src, const float *sscale, float *dst, int n12) {
+  const int z = blockIdx.y;
+  const long base = (long)z * n12;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += gridDim.x * blockDim.x)
+    dst[base + i] = snap_dyn(fmt, src, sscale, (int)(base + i), z);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 dot product: fixed grid, per-block partials, single-block finish, so
+// the result is deterministic run to run.
+// ---------------------------------------------------------------------------
+__global__ void k_dot_partial(const float *__restrict__ a, const float *__restrict__ b, long n,
+                              double *partials) {
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void k_dot_final(const double *partials, int np, double *out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_axpby(float *y, const float *__restrict__ x, double a, double b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = (float)(a * (double)y[i] + b * (double)x[i]);
+}
+
+__global__ void k_mul(float *y, const float *__restrict__ x, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] *= x[i];
+}
+
+__global__ void k_fill(float *y, float v, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+// D_z (Born/laplac_3d.cpp:46-65 forward, :4-22 adjoint) on [n3][n12].
+__global__ void k_dz(int adj, const float *__restrict__ in, float *out, int n12, int n3,
+                     int add) {
+  const long n = (long)n12 * n3;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / n12);
+    float v;
+    if (!adj) {  // d[z] = m[z-1] (z >= 1) - m[z] (z <= n3-2)
+      v = (z >= 1 ? in[i - n12] : 0.f) - (z <= n3 - 2 ? in[i] : 0.f);
+    } else {     // m[z] = d[z+1] - d[z] for z <= n3-2
+      v = z <= n3 - 2 ? in[i + n12] - in[i] : 0.f;
+    }
+    out[i] = add ? out[i] + v : v;
+  }
+}
+
+__global__ void k_embed(int adj, const float *model, float *comp, int m1, int m2, int m3, int n1,
+                        int n2, int n3, int o1, int o2, int o3) {
+  const long n = (long)n1 * n2 * n3;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int i1 = (int)(i % n1);
+    const int i2 = (int)((i / n1) % n2);
+    const int i3 = (int)(i / ((long)n1 * n2));
+    const int j1 = i1 - o1, j2 = i2 - o2, j3 = i3 - o3;
+    const bool ok = j1 >= 0 && j1 < m1 && j2 >= 0 && j2 < m2 && j3 >= 0 && j3 < m3;
+    const long j = ((long)j3 * m2 + j2) * m1 + j1;
+    if (!adj) {
+      comp[i] = ok ? model[j] : 0.f;
+    } else if (ok) {
+      const_cast<float *>(model)[j] += comp[i];
+    }
+  }
+}
+
+// Device traces [ntb][nrec] -> data [nrec][nt]:
+//   w[t] = tr[t+4];  d[t] = (w[t-1] - w[t]) * inv_dt   (w[-1] = 0).
+// 32x32 LDS tiles so both sides are coalesced.
+__global__ void k_trace_to_data(const float *__restrict__ tr, float *data, int nrec, int ntb,
+                                int nt, float inv_dt, int add) {
+  __shared__ float tile[33][33];
+  const int r0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, tyy = threadIdx.y;  // 32 x 8
+  // load w[t0-1 .. t0+31] for 32 receivers
+  for (int k = tyy; k < 33; k += 8) {
+    const int t = t0 - 1 + k;
+    const int r = r0 + tx;
+    float v = 0.f;
+    if (r < nrec && t >= 0 && t < nt) v = tr[(long)(t + 4) * nrec + r];
+    tile[k][tx] = v;
+  }
+  __syncthreads();
+  for (int k = tyy; k < 32; k += 8) {
+    const int r = r0 + k;
+    const int t = t0 + tx;
+    if (r < nrec && t < nt) {
+      const float d = (tile[tx][k] - tile[tx + 1][k]) * inv_dt;
+      float *o = data + (long)r * nt + t;
+      *o = add ? *o + d : d;
+    }
+  }
+}
+
+// Adjoint of k_trace_to_data: tr[t+4] = (d[t+1] - d[t]) * inv_dt (d[nt] = 0),
+// other samples of tr zero.
+__global__ void k_data_to_trace(const float *__restrict__ data, float *tr, int nrec, int ntb,
+                                int nt, float inv_dt) {
+  __shared__ float tile[33][33];
+  const int r0 = blockIdx.x * 32, t0 = blockIdx.y * 32;  // t0 indexes trace samples
+  const int tx = threadIdx.x, tyy = threadIdx.y;
+  // data samples t = t0-4 .. t0+28 (33 values) for 32 receivers
+  for (int k = tyy; k < 32; k += 8) {
+    const int r = r0 + k;
+    for (int kk = tx; kk < 33; kk += 32) {
+      const int t = t0 - 4 + kk;
+      float v = 0.f;
+      if (r < nrec && t >= 0 && t < nt) v = data[(long)r * nt + t];
+      tile[kk][k] = v;
+    }
+  }
+  __syncthreads();
+  for (int k = tyy; k < 32; k += 8) {
+    const int ts = t0 + k;  // trace sample
+    const int r = r0 + tx;
+    if (r < nrec && ts < ntb) {
+      const int t = ts - 4;
+      float v = 0.f;
+      if (t >= 0 && t < nt) v = (tile[k + 1][tx] - tile[k][tx]) * inv_dt;
+      tr[(long)ts * nrec + r] = v;
+    }
+  }
+}
+
+__global__ void k_transpose(const float *__restrict__ in, float *out, long rows, long cols) {
+  __shared__ float tile[32][33];
+  const long c0 = (long)blockIdx.x * 32, r0 = (long)blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const long r = r0 + k, c = c0 + threadIdx.x;
+    if (r < rows && c < cols) tile[k][threadIdx.x] = in[r * cols + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const long c = c0 + k, r = r0 + threadIdx.x;
+    if (r < rows && c < cols) out[c * rows + r] = tile[threadIdx.x][k];
+  }
+}
+
+int grid_for(long n, int bs) {
+  long g = (n + bs - 1) / bs;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <int FORM, int TY, int RPT>
+hipError_t launch_step_t(int fmt, const StepParams &P, hipStream_t st) {
+  const dim3 block(64, TY / RPT);
+  const dim3 grid(P.ntx * P.nty * P.ntz);
+  switch (fmt) {
+    case FMT_NONE: k_step<FORM, TY, RPT, FMT_NONE><<<grid, block, 0, st>>>(P); break;
+    case FMT_F32: k_step<FORM, TY, RPT, FMT_F32><<<grid, block, 0, st>>>(P); break;
+    case FMT_BF16: k_step<FORM, TY, RPT, FMT_BF16><<<grid, block, 0, st>>>(P); break;
+    case FMT_F16: k_step<FORM, TY, RPT, FMT_F16><<<grid, block, 0, st>>>(P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+void plan_step(StepParams &P, int ty_tile, int target_blocks) {
+  P.ntx = (P.n1 + 63) / 64;
+  P.nty = (P.n2 + ty_tile - 1) / ty_tile;
+  const int nz = P.n3 - 8;
+  const long tiles = (long)P.ntx * P.nty;
+  long zc = (tiles * nz + target_blocks - 1) / target_blocks;
+  if (zc < 8) zc = 8;
+  if (zc > nz) zc = nz;
+  P.zc = (int)zc;
+  P.ntz = (nz + P.zc - 1) / P.zc;
+}
+
+hipError_t launch_step(int form, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+  if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 9) return hipErrorInvalidValue;
+  if (form == FORM_VL) {
+    if (ty_tile == 32) return launch_step_t<FORM_VL, 32, 4>(fmt, P, st);
+    if (ty_tile == 8) return launch_step_t<FORM_VL, 8, 2>(fmt, P, st);
+    return launch_step_t<FORM_VL, 16, 4>(fmt, P, st);
+  }
+  if (ty_tile == 32) return launch_step_t<FORM_LV, 32, 4>(fmt, P, st);
+  if (ty_tile == 8) return launch_step_t<FORM_LV, 8, 2>(fmt, P, st);
+  return launch_step_t<FORM_LV, 16, 4>(fmt, P, st);
+}
+
+hipError_t launch_inject(const InjectParams &P, hipStream_t st) {
+  if (P.npts <= 0) return hipSuccess;
+  k_inject<<<(P.npts + 255) / 256, 256, 0, st>>>(P);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract(int npts, const int *locs, float *tr, long st_t, int id,
+                          const float s[8], const float *p, hipStream_t st) {
+  if (npts <= 0) return hipSuccess;
+  k_extract<<<(npts + 255) / 256, 256, 0, st>>>(npts, locs, tr, st_t, id, s[0], s[1], s[2], s[3],
+                                                s[4], s[5], s[6], s[7], p);
+  return hipGetLastError();
+}
+
+hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int n12, int n3,
+                         hipStream_t st) {
+  hipError_t e = hipMemsetAsync(sscale, 0, sizeof(float) * n3, st);
+  if (e != hipSuccess) return e;
+  int gx = (n12 + 255) / 256;
+  if (gx > 64) gx = 64;
+  k_slice_absmax<<<dim3(gx, n3), 256, 0, st>>>(s, reinterpret_cast<unsigned int *>(sscale), n12);
+  if (fmt == FMT_BF16)
+    k_encode<FMT_BF16><<<dim3(gx, n3), 256, 0, st>>>(s, dst, sscale, n12);
+  else if (fmt == FMT_F16)
+    k_encode<FMT_F16><<<dim3(gx, n3), 256, 0, st>>>(s, dst, sscale, n12);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_decode(int fmt, const void *src, const float *sscale, float *dst, int n12,
+                         int n3, hipStream_t st) {
+  int gx = (n12 + 255) / 256;
+  if (gx > 64) gx = 64;
+  k_decode<<<dim3(gx, n3), 256, 0, st>>>(fmt, src, sscale, dst, n12);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot(const float *a, const float *b, long n, double *partials, double *out,
+                      hipStream_t st) {
+  k_dot_partial<<<kDotBlocks, 256, 0, st>>>(a, b, n, partials);
+  k_dot_final<<<1, 256, 0, st>>>(partials, kDotBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(float *y, const float *x, double a, double b, long n, hipStream_t st) {
+  k_axpby<<<grid_for(n, 256), 256, 0, st>>>(y, x, a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mul(float *y, const float *x, long n, hipStream_t st) {
+  k_mul<<<grid_for(n, 256), 256, 0, st>>>(y, x, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float *y, float v, long n, hipStream_t st) {
+  k_fill<<<grid_for(n, 256), 256, 0, st>>>(y, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_dz(int adj, const float *in, float *out, int n12, int n3, int add,
+                     hipStream_t st) {
+  k_dz<<<grid_for((long)n12 * n3, 256), 256, 0, st>>>(adj, in, out, n12, n3, add);
+  return hipGetLastError();
+}
+
+hipError_t launch_embed(int adj, const float *model, float *comp, int m1, int m2, int m3, int n1,
+                        int n2, int n3, int o1, int o2, int o3, hipStream_t st) {
+  k_embed<<<grid_for((long)n1 * n2 * n3, 256), 256, 0, st>>>(adj, model, comp, m1, m2, m3, n1,
+                                                             n2, n3, o1, o2, o3);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_to_data(const float *tr, float *data, int nrec, int ntb, int nt,
+                                float inv_dt, int add, hipStream_t st) {
+  dim3 grid((nrec + 31) / 32, (nt + 31) / 32);
+  k_trace_to_data<<<grid, dim3(32, 8), 0, st>>>(tr, data, nrec, ntb, nt, inv_dt, add);
+  return hipGetLastError();
+}
+
+hipError_t launch_data_to_trace(const float *data, float *tr, int nrec, int ntb, int nt,
+                                float inv_dt, hipStream_t st) {
+  dim3 grid((nrec + 31) / 32, (ntb + 31) / 32);
+  k_data_to_trace<<<grid, dim3(32, 8), 0, st>>>(data, tr, nrec, ntb, nt, inv_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const float *in, float *out, long rows, long cols, hipStream_t st) {
+  dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  k_transpose<<<grid, dim3(32, 8), 0, st>>>(in, out, rows, cols);
+  return hipGetLastError();
+}
+
+}  // namespace brtm

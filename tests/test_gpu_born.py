@@ -1,0 +1,196 @@
+"""GPU parity of the HIP Born path against the FP64 oracle (through the C ABI).
+
+Tolerances (stated by the north star): wavefields within 1e-5 relative L2 of
+the FP64 oracle; images and adjoint dot-product tests within 1e-4."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import born_oracle as O
+from sep_code_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def announce(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def random_field(r, n, scale=1.0):
+    n1, n2, n3 = n
+    p = np.zeros((n3, n2, n1))
+    p[4:-4, 4:-4, 4:-4] = r.standard_normal((n3 - 8, n2 - 8, n1 - 8)) * scale
+    return p
+
+
+def velocity(r, n, d, dt=0.001, vlo=1500.0, vhi=3500.0):
+    n1, n2, n3 = n
+    v = r.uniform(vlo, vhi, (n3, n2, n1))
+    return (v * dt) ** 2
+
+
+@pytest.mark.parametrize("n", [(70, 36, 40), (128, 24, 30), (64, 64, 20)])
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("ty", [8, 16, 32])
+def test_step_parity(dev, n, form, ty):
+    announce(f"step parity n={n} form={form} ty={ty}")
+    r = np.random.default_rng(11)
+    d = (10.0, 11.0, 12.0)
+    V = velocity(r, n, d)
+    bc_a, bc_b = 7.0, 0.03
+    dev.set_tuning(ty, 2048)
+    dev.create_space(d, bc_a, bc_b, n)
+    dev.set_velocity(V)
+    P = O.Problem(n, d, V, bc_a, bc_b)
+    p1, p0 = random_field(r, n), random_field(r, n)
+    # one step: only FP32 rounding separates the two
+    g_cur, g_old = dev.propagate(form, 1, p1, p0)
+    o_cur, o_old = P.propagate(form, 1, p1.ravel(), p0.ravel())
+    assert rel(g_cur, o_cur) < 1e-6
+    np.testing.assert_array_equal(g_old.ravel(), p1.ravel().astype(np.float32))
+    # 60 steps
+    g_cur, g_old = dev.propagate(form, 60, p1, p0)
+    o_cur, o_old = P.propagate(form, 60, p1.ravel(), p0.ravel())
+    assert rel(g_cur, o_cur) < 1e-5
+    assert rel(g_old, o_old) < 1e-5
+    # the zero halo is never written
+    U = np.zeros(g_cur.shape, bool)
+    U[4:-4, 4:-4, 4:-4] = True
+    assert np.all(g_cur[~U] == 0)
+    dev.set_tuning(16, 2048)
+
+
+def make_shot(dev, n=(48, 44, 40), nt=120, jt=1, jimg=1, seed=3, cimg=1e-3, bc_a=8.0,
+              bc_b=0.02, d=(10.0, 10.0, 10.0), rec_step=3):
+    """Same shot on the device and in the oracle."""
+    r = np.random.default_rng(seed)
+    n1, n2, n3 = n
+    dt = 0.001
+    V = velocity(r, n, d, dt)
+    ntr = (nt - 1) // jt + 9
+    w = G.wavelet_derivative(G.ricker(25.0, dt * jt, ntr - 8), dt * jt)
+    slocs, strc = G.source_points(n1, n2, n1 / 2 + 0.3, n2 / 2 - 0.4, 9.5, w, ntr)
+    rlocs, nx, ny = G.surface_receivers(n, 4, 3.0, step=rec_step, origin=1)
+    sinc = O.sinc_table(jt).astype(np.float32)
+    dev.create_space(d, bc_a, bc_b, n)
+    dev.set_velocity(V)
+    dev.set_imaging_scale(cimg)
+    dev.set_sinc_table(0, sinc)
+    dev.set_sinc_table(1, sinc)
+    dev.set_source(slocs, strc)
+    dev.set_receivers(rlocs, None, ntr=ntr, nx=nx, ny=ny)
+    # inactive receivers (outside the update set) are dropped on both sides
+    rl = rlocs.copy()
+    i1, i2, i3 = rl % n1, (rl // n1) % n2, rl // (n1 * n2)
+    inside = (i1 >= 4) & (i1 < n1 - 4) & (i2 >= 4) & (i2 < n2 - 4) & (i3 >= 4) & (i3 < n3 - 4)
+    rl[~inside] = -1
+    P = O.Problem(n, d, V, bc_a, bc_b, src_locs=slocs, src_tr=strc.astype(np.float64),
+                  sinc_s=sinc.astype(np.float64), rec_locs=rl, ntr_d=ntr,
+                  sinc_d=sinc.astype(np.float64), jt=jt, nt=nt, jimg=jimg, cimg=cimg)
+    return P, r
+
+
+@pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 3)])
+def test_source_prop_parity(dev, jt, jimg):
+    announce(f"source_prop parity jt={jt} jimg={jimg}")
+    P, r = make_shot(dev, jt=jt, jimg=jimg)
+    p0, p1 = dev.source_prop(jt, P.nt, jimg, store="f32", get_last=True)
+    snaps, last2 = P.source_prop()
+    assert rel(p1, last2[1]) < 1e-5 and rel(p0, last2[0]) < 1e-5
+    for k in (0, P.nimg // 2, P.nimg - 1):
+        assert rel(dev.get_snapshot(k), snaps[k]) < 1e-5, k
+
+
+@pytest.mark.parametrize("store", ["f32", "none", "ckpt"])
+def test_born_forward_parity(dev, store):
+    announce(f"born_forward parity store={store}")
+    P, r = make_shot(dev, jimg=2)
+    dev.source_prop(1, P.nt, 2, store=store, ckpt=17)
+    m = random_field(r, P.n)
+    dev.set_model(m)
+    dev.rtm_forward(1, P.nt, 2)
+    D = dev.get_receivers()
+    snaps, _ = P.source_prop()
+    Do = P.born_forward(m.ravel(), snaps)
+    assert rel(D, Do) < 1e-5
+
+
+@pytest.mark.parametrize("store", ["f32", "ckpt"])
+def test_rtm_adjoint_parity(dev, store):
+    announce(f"rtm_adjoint parity store={store}")
+    P, r = make_shot(dev, jimg=2)
+    dev.source_prop(1, P.nt, 2, store=store, ckpt=13)
+    snaps, _ = P.source_prop()
+    m = random_field(r, P.n)
+    Dobs = P.born_forward(m.ravel(), snaps)
+    dev.set_receivers(P.rlocs, Dobs)
+    dev.zero_image()
+    dev.rtm_adjoint(1, P.nt, 2)
+    img = dev.get_image()
+    io = P.rtm_adjoint(Dobs, snaps)
+    assert rel(img, io) < 1e-4
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16", "f16", "ckpt"])
+def test_dot_product_gpu(dev, store):
+    """<F m, d> = <m, F^T d> within 1e-4 with the FP32 GPU operator, for every
+    source-wavefield store (both directions read the same snapshots)."""
+    announce(f"dot test store={store}")
+    P, r = make_shot(dev, n=(40, 36, 44), nt=100, jimg=1 if store != "f16" else 2)
+    jimg = P.jimg
+    dev.source_prop(1, P.nt, jimg, store=store, ckpt=9)
+    m = random_field(r, P.n)
+    dev.set_model(m)
+    dev.rtm_forward(1, P.nt, jimg)
+    Fm = dev.get_receivers().astype(np.float64)
+    d = r.standard_normal(Fm.shape)
+    d[P.rlocs < 0] = 0
+    dev.set_receivers(P.rlocs, d)
+    dev.zero_image()
+    dev.rtm_adjoint(1, P.nt, jimg)
+    Ftd = dev.get_image().astype(np.float64)
+    lhs, rhs = (Fm * d).sum(), (m * Ftd).sum()
+    assert abs(lhs - rhs) / abs(lhs) < 1e-4, (lhs, rhs)
+
+
+@pytest.mark.parametrize("store,tol", [("bf16", 2e-2), ("f16", 2e-3)])
+def test_compressed_store_image_error(dev, store, tol):
+    """Image from BF16 / FP16 snapshots (per-z-slice FP32 scale) against the
+    FP64 oracle with exact snapshots: error bounded by the format."""
+    announce(f"compressed image error store={store}")
+    P, r = make_shot(dev, jimg=4)
+    dev.source_prop(1, P.nt, 4, store=store)
+    snaps, _ = P.source_prop()
+    m = random_field(r, P.n)
+    Dobs = P.born_forward(m.ravel(), snaps)
+    dev.set_receivers(P.rlocs, Dobs)
+    dev.zero_image()
+    dev.rtm_adjoint(1, P.nt, 4)
+    e = rel(dev.get_image(), P.rtm_adjoint(Dobs, snaps))
+    announce(f"  image rel err {store} = {e:.3e}")
+    assert e < tol
+
+
+def test_cfl_violation_refused(dev):
+    from sep_code_amd._lib import BrtmError
+    n = (32, 32, 32)
+    dev.create_space((10.0, 10.0, 10.0), 4.0, 0.01, n)
+    V = np.full((32, 32, 32), (5000 * 0.001) ** 2, np.float32)  # v dt / d = 0.5
+    with pytest.raises(BrtmError, match="stability"):
+        dev.set_velocity(V)
+    dev.set_velocity(V * 0.8)  # 0.447 < 0.4529
+
+
+def test_adjoint_requires_store(dev):
+    from sep_code_amd._lib import BrtmError
+    P, r = make_shot(dev, nt=20)
+    dev.source_prop(1, 20, 1, store="none")
+    with pytest.raises(BrtmError, match="stored source"):
+        dev.rtm_adjoint(1, 20, 1)

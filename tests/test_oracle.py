@@ -1,0 +1,354 @@
+"""Pins of the CPU oracle (oracle/) -- each against something other than the
+oracle's own formula: closed forms, library routines, the reference's own code
+built from its sources (oracle/_ref), adjoint identities or brute force."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import born_oracle as O
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def small_problem(n=(20, 22, 24), d=(10.0, 12.0, 14.0), bc_a=6, bc_b=0.02, seed=0, prec=64,
+                  **kw):
+    r = rng(seed)
+    n1, n2, n3 = n
+    V = r.uniform(0.05, 0.12, (n3, n2, n1)) * min(d) ** 2 / 4.0
+    return O.Problem(n, d, V, bc_a, bc_b, prec=prec, **kw), r
+
+
+# --- A1 coefficients -------------------------------------------------------
+def test_coeffs_are_fornberg_weights():
+    """Born/wave_fkernel.3d8o.cu:26-27 equals the textbook 8th-order second
+    derivative weights {-205/72, 8/5, -1/5, 8/315, -1/560}/h^2 per axis."""
+    d = (3.0, 5.0, 7.0)
+    c = O.coeffs(*d)
+    fw = [8 / 5, -1 / 5, 8 / 315, -1 / 560]
+    assert np.isclose(c[0], sum(-205 / 72 / h ** 2 for h in d), rtol=1e-14)
+    for k in range(4):
+        for a in range(3):
+            assert np.isclose(c[1 + 3 * k + a], fw[k] / d[a] ** 2, rtol=1e-14)
+
+
+def test_1d_weights_exact_on_polynomials():
+    """The 1-D weights differentiate x^p exactly for p <= 9 (order 8)."""
+    w = np.array([-1 / 560, 8 / 315, -1 / 5, 8 / 5, -205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560])
+    x0, h = 0.3, 0.1
+    xs = x0 + h * np.arange(-4, 5)
+    for p in range(10):
+        approx = (w * xs ** p).sum() / h ** 2
+        exact = p * (p - 1) * x0 ** (p - 2) if p >= 2 else 0.0
+        assert abs(approx - exact) < 1e-8 * max(1, abs(exact)), p
+    approx = (w * xs ** 10).sum() / h ** 2
+    assert abs(approx - 90 * x0 ** 8) > 1e-6  # degree 10 is not exact
+
+
+def test_stencil_laplacian_of_polynomial_anisotropic():
+    """With w = 1, V = 1, p_old = 0 the VL step returns L p + 2p; for a
+    quartic in each axis L8 p is the exact Laplacian (catches axis swaps)."""
+    n, d = (18, 19, 20), (2.0, 3.0, 5.0)
+    P = O.Problem(n, d, np.ones((n[2], n[1], n[0])), 0.0, 0.0)
+    i3, i2, i1 = np.meshgrid(*[np.arange(k, dtype=np.float64) for k in n[::-1]], indexing="ij")
+    x, y, z = i1 * d[0], i2 * d[1], i3 * d[2]
+    p = 0.5 * x ** 4 + 2 * y ** 3 - 0.25 * z ** 4 + x * y * z + 3 * y ** 2 * z
+    lap = 6 * x ** 2 + 12 * y - 3 * z ** 2 + 6 * z
+    pn = P.step_vl(p.ravel(), np.zeros(p.size)).reshape(p.shape)
+    U = (slice(4, -4),) * 3
+    np.testing.assert_allclose((pn - 2 * p)[U], lap[U], rtol=1e-9, atol=1e-6)
+    # outside the update set nothing is written
+    mask = np.ones(p.shape, bool)
+    mask[U] = False
+    assert np.all(pn[mask] == 0)
+
+
+def test_stencil_symbol_on_plane_wave():
+    """L8 cos(k.x) = sigma(k) cos(k.x) at interior points, sigma the discrete
+    symbol sum_a (c0_a + 2 sum_k c_ak cos(k theta_a))."""
+    n, d = (24, 24, 24), (10.0, 10.0, 10.0)
+    P = O.Problem(n, d, np.ones((24, 24, 24)), 0.0, 0.0)
+    th = (0.7, 1.3, 2.1)
+    i3, i2, i1 = np.meshgrid(*[np.arange(24.0)] * 3, indexing="ij")
+    p = np.cos(th[0] * i1 + th[1] * i2 + th[2] * i3)
+    fw = [8 / 5, -1 / 5, 8 / 315, -1 / 560]
+    sig = sum((-205 / 72 + 2 * sum(fw[k] * math.cos((k + 1) * t) for k in range(4))) / 100.0
+              for t in th)
+    pn = P.step_vl(p.ravel(), np.zeros(p.size)).reshape(p.shape)
+    U = (slice(4, -4),) * 3
+    np.testing.assert_allclose((pn - 2 * p)[U], sig * p[U], atol=1e-12)
+
+
+# --- A4 sponge -------------------------------------------------------------
+def test_sponge_table_closed_form():
+    w = O.sponge_table(50.0, 0.0005)
+    assert len(w) == 50
+    for e in (0, 10, 49):
+        assert np.isclose(w[e], math.exp(-0.0005 * (50 - e)), rtol=1e-15)
+    assert np.all(np.diff(w) > 0) and np.all(w < 1)
+
+
+def test_sponge_edge_distance_matches_reference_formula():
+    """w on a ones field (V = 0, p_old = 0: p' = 2w).  Hand-evaluated cells of
+    Born/wave_fkernel.3d8o.cu:194: e = min(i-4, n-4-i) per axis, so the low
+    face's first updated cell has e = 0 and the high face's last one e = 1."""
+    n = (20, 21, 22)
+    a, b = 5.0, 0.1
+    P = O.Problem(n, (1.0, 1.0, 1.0), np.zeros((22, 21, 20)), a, b)
+    p = np.ones(20 * 21 * 22)
+    w = P.step_vl(p, np.zeros_like(p)).reshape(22, 21, 20) / 2
+    ex = lambda e: math.exp(-b * (a - e)) if e < a else 1.0
+    assert np.isclose(w[11, 10, 4], ex(0))          # i1 = 4
+    assert np.isclose(w[11, 10, 15], ex(1))         # i1 = n1-5 -> n1-4-i1 = 1
+    assert np.isclose(w[11, 4, 10], ex(0))          # i2 = 4
+    assert np.isclose(w[17, 10, 10], ex(1))         # i3 = n3-5
+    assert np.isclose(w[6, 7, 8], ex(2))            # min(4, 3, 2) = 2
+    assert np.isclose(w[11, 10, 10], 1.0)           # deep interior
+
+
+# --- A2/A3: the LV step is the exact transpose of the VL step ---------------
+def _step_matrix(P, form):
+    n1, n2, n3 = P.n
+    N = n1 * n2 * n3
+    U = np.zeros((n3, n2, n1), bool)
+    U[4:-4, 4:-4, 4:-4] = True
+    idx = np.flatnonzero(U.ravel())
+    A = np.zeros((len(idx), len(idx)))
+    B = np.zeros((len(idx), len(idx)))
+    for j, cj in enumerate(idx):
+        e = np.zeros(N)
+        e[cj] = 1.0
+        f = P.step_vl if form == 0 else P.step_lv
+        A[:, j] = f(e, np.zeros(N))[idx]
+        B[:, j] = -f(np.zeros(N), e)[idx]
+    return A, B
+
+
+def test_lv_step_is_transpose_of_vl_step():
+    P, _ = small_problem(n=(10, 11, 12), d=(1.0, 1.0, 1.0), bc_a=3.0, bc_b=0.2)
+    P.V[:] = np.random.default_rng(3).uniform(0.01, 0.08, P.V.size)
+    A_vl, B_vl = _step_matrix(P, 0)
+    A_lv, B_lv = _step_matrix(P, 1)
+    assert np.abs(A_vl - A_vl.T).max() > 1e-3  # VL alone is not symmetric
+    np.testing.assert_allclose(A_lv, A_vl.T, atol=1e-14)
+    np.testing.assert_allclose(B_lv, B_vl.T, atol=1e-15)
+    assert np.allclose(np.diag(B_vl), np.diag(B_vl)) and np.all(np.diag(B_vl) <= 1)
+
+
+def test_undamped_leapfrog_is_time_reversible():
+    """Without sponge and sources p' = V L p + 2p - p_old is exactly
+    reversible: stepping back from (p_N, p_{N-1}) recovers (p_0, p_1)."""
+    P, r = small_problem(bc_a=0.0, bc_b=0.0)
+    p0 = np.zeros(P.ncell)
+    p1 = np.zeros(P.ncell)
+    U = np.zeros(P.ncell, bool).reshape(24, 22, 20)
+    U[4:-4, 4:-4, 4:-4] = True
+    p1[U.ravel()] = r.standard_normal(U.sum())
+    p0[U.ravel()] = r.standard_normal(U.sum())
+    cur, old = P.propagate(0, 40, p1, p0)   # cur = p_41, old = p_40
+    back_cur, back_old = P.propagate(0, 40, old, cur)
+    np.testing.assert_allclose(back_cur, p0, atol=1e-9)
+    np.testing.assert_allclose(back_old, p1, atol=1e-9)
+
+
+def test_sponge_dissipates_energy():
+    P, r = small_problem(n=(32, 32, 32), bc_a=8.0, bc_b=0.05)
+    p = np.zeros(P.ncell)
+    p.reshape(32, 32, 32)[16, 16, 16] = 1.0
+    cur, old = P.propagate(0, 300, p, np.zeros(P.ncell))
+    P0, _ = small_problem(n=(32, 32, 32), bc_a=0.0, bc_b=0.0)
+    c0, o0 = P0.propagate(0, 300, p, np.zeros(P.ncell))
+    assert np.linalg.norm(cur) < 0.5 * np.linalg.norm(c0)
+
+
+# --- A5-A7 injection / extraction ------------------------------------------
+def test_inject_extract_adjoint_pair():
+    r = rng(5)
+    N, npt, ntr = 500, 7, 30
+    locs = r.choice(N, npt, replace=False).astype(np.int32)
+    s8 = r.standard_normal(8)
+    p = r.standard_normal(N)
+    tr = r.standard_normal((npt, ntr))
+    L = O.lib(64)
+    inj = np.zeros(N)
+    L.orc_inject(npt, locs.ctypes.data, tr.ctypes.data, ntr, s8.ctypes.data, 5, 1.0,
+                 inj.ctypes.data)
+    ext = np.zeros((npt, ntr))
+    L.orc_extract(npt, locs.ctypes.data, ext.ctypes.data, ntr, s8.ctypes.data, 5, p.ctypes.data)
+    assert np.isclose(inj @ p, (ext * tr).sum(), rtol=1e-13)
+    # closed form for one point
+    assert np.isclose(inj[locs[0]], (s8 * tr[0, 5:13]).sum())
+
+
+# --- Born forward / RTM adjoint ----------------------------------------------
+def born_case(n=(24, 26, 28), nt=60, jt=1, jimg=1, seed=1, prec=64, bc_a=6.0, bc_b=0.03):
+    r = rng(seed)
+    n1, n2, n3 = n
+    d = (10.0, 10.0, 10.0)
+    V = (r.uniform(1500, 3000, (n3, n2, n1)) * 0.001 / 10.0) ** 2 * 100.0 / 100.0
+    V = (r.uniform(1500, 3000, (n3, n2, n1)) * 0.001) ** 2
+    ntr = (nt - 1) // jt + 9
+    w = O.wavelet_derivative(np.sin(np.linspace(0, 6, ntr - 8)) * np.hanning(ntr - 8), 0.001)
+    slocs, str_ = O.source_geometry(n1, n2, 9.3, n1 / 2 + 0.4, n2 / 2 - 0.2, w, ntr)
+    xs, ys = np.meshgrid(np.arange(5, n1 - 5, 3), np.arange(5, n2 - 5, 4))
+    rl = O.receiver_index(n1, n2, xs.ravel() + 0.5, ys.ravel(), np.full(xs.size, 6.2))
+    sinc = O.sinc_table(jt) if jt > 1 else None
+    P = O.Problem(n, d, V, bc_a, bc_b, src_locs=slocs, src_tr=str_, sinc_s=sinc,
+                  rec_locs=rl.astype(np.int32), ntr_d=ntr, sinc_d=sinc, jt=jt, nt=nt, jimg=jimg,
+                  cimg=1e-3, prec=prec)
+    return P, r
+
+
+@pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 3)])
+def test_born_dot_product_fp64(jt, jimg):
+    """<F m, d> = <m, F^T d> to FP64 round-off (my_operator::dot_test,
+    Born/my_operator.cpp:14-76, at 1e-12 instead of 1e-4)."""
+    P, r = born_case(jt=jt, jimg=jimg)
+    snaps, _ = P.source_prop()
+    m = r.standard_normal(P.ncell)
+    d = r.standard_normal((len(P.rlocs), P.ntr_d))
+    Fm = P.born_forward(m, snaps)
+    Ftd = P.rtm_adjoint(d, snaps)
+    lhs, rhs = (Fm * d).sum(), (m * Ftd).sum()
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+    assert abs(lhs) > 0
+
+
+def test_born_forward_live_source_equals_snapshots():
+    P, r = born_case(jimg=2)
+    snaps, _ = P.source_prop()
+    m = r.standard_normal(P.ncell)
+    np.testing.assert_array_equal(P.born_forward(m, snaps), P.born_forward(m, None))
+
+
+def test_adjoint_checkpoint_equals_snapshots():
+    P, r = born_case(nt=50, jimg=2)
+    snaps, _ = P.source_prop()
+    d = r.standard_normal((len(P.rlocs), P.ntr_d))
+    a = P.rtm_adjoint(d, snaps)
+    for ck in (1, 7, 50, 64):
+        np.testing.assert_array_equal(P.rtm_adjoint(d, None, ckpt=ck), a)
+
+
+def test_born_is_linear_in_model():
+    P, r = born_case(nt=40)
+    snaps, _ = P.source_prop()
+    m1, m2 = r.standard_normal(P.ncell), r.standard_normal(P.ncell)
+    np.testing.assert_allclose(P.born_forward(2 * m1 - 3 * m2, snaps),
+                               2 * P.born_forward(m1, snaps) - 3 * P.born_forward(m2, snaps),
+                               rtol=1e-10, atol=1e-12)
+
+
+def test_fp32_build_agrees_with_fp64():
+    P64, r = born_case(nt=40)
+    P32, _ = born_case(nt=40, prec=32)
+    s64, l64 = P64.source_prop()
+    s32, l32 = P32.source_prop()
+    rel = np.linalg.norm(l32 - l64) / np.linalg.norm(l64)
+    assert rel < 1e-5, rel
+
+
+# --- host-side operators ---------------------------------------------------
+def test_dz_dt_window_are_adjoint_pairs():
+    r = rng(7)
+    m = r.standard_normal((9, 5, 6))
+    d = r.standard_normal((9, 5, 6))
+    assert np.isclose((O.dz_forward(m) * d).sum(), (m * O.dz_adjoint(d)).sum(), rtol=1e-13)
+    tr = r.standard_normal((4, 3, 40))
+    dd = r.standard_normal((4, 3, 40))
+    assert np.isclose((O.dt_forward(tr, 0.004) * dd).sum(), (tr * O.dt_adjoint(dd, 0.004)).sum(),
+                      rtol=1e-13)
+    wm = r.standard_normal((4, 48))
+    wd = r.standard_normal((4, 40))
+    assert np.isclose((O.window_forward(wm, 40) * wd).sum(),
+                      (wm * O.window_adjoint(wd, 48)).sum(), rtol=1e-13)
+
+
+def test_dz_and_dt_closed_forms():
+    z = np.arange(10.0)[:, None, None] * np.ones((10, 2, 3))
+    dz = O.dz_forward(z)   # d[i] = m[i-1] - m[i] = -1 for 1 <= i <= n-2
+    assert np.allclose(dz[1:-1], -1.0) and np.allclose(dz[0], 0.0) and np.allclose(dz[-1], 8.0)
+    t = np.arange(20.0) * 0.5
+    dt = O.dt_forward(t, 0.5)  # (m[t-1]-m[t])/dt = -1, first sample -m[0]/dt = 0
+    assert np.allclose(dt[1:], -1.0) and dt[0] == 0.0
+
+
+# --- A14 interpolation tables ----------------------------------------------
+def test_levinson_matches_scipy_toeplitz():
+    r = rng(2)
+    col = np.array([4.0, 1.5, 0.7, 0.2, 0.1, 0.05, 0.02, 0.01])
+    g = r.standard_normal(8)
+    np.testing.assert_allclose(O.levinson(col, g), scipy.linalg.solve_toeplitz(col, g),
+                               rtol=1e-10)
+
+
+def test_sinc_zero_shift_is_identity_and_interpolates():
+    t = O.sinc_table(4)
+    np.testing.assert_allclose(t[0], np.eye(8)[3], atol=1e-9)
+    # interpolates a low-frequency sinusoid at fractional shift d
+    for i, dfr in enumerate((0.25, 0.5, 0.75)):
+        x = np.arange(8) - 3.0
+        f = 0.05
+        y = np.sin(2 * np.pi * f * x)
+        assert abs((t[i + 1] * y).sum() - np.sin(2 * np.pi * f * dfr)) < 2e-3
+
+
+@pytest.mark.parametrize("ntab", [1, 2, 4, 5])
+def test_sinc_matches_reference_sinc_bob(ntab):
+    """Against Born/sinc_bob.cpp built from the reference sources: the FP64
+    table agrees to 1e-4 (measured gap <= 4.2e-5 at ntab = 5; the reference
+    sums its Toeplitz terms and runs its Levinson vectors in float)."""
+    ref = O.ref_lib("ref_sinc")
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    import ctypes as C
+    out = np.zeros((ntab, 8), np.float32)
+    ref.ref_sinc_table(ntab, 8, C.c_void_p(out.ctypes.data))
+    np.testing.assert_allclose(O.sinc_table(ntab), out, atol=1e-4)
+
+
+# --- A24/A25 CG --------------------------------------------------------------
+def test_cgstep_converges_to_lstsq():
+    r = rng(4)
+    A = r.standard_normal((40, 12))
+    d = r.standard_normal(40)
+    m, res = O.cgstep_solve(A, d, 12)
+    np.testing.assert_allclose(m, np.linalg.lstsq(A, d, rcond=None)[0], rtol=1e-7, atol=1e-9)
+    assert all(res[i + 1] <= res[i] * (1 + 1e-12) for i in range(len(res) - 1))
+
+
+def test_cgstep_matches_reference_cgstep():
+    ref = O.ref_lib("ref_cgstep")
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    import ctypes as C
+    r = rng(6)
+    A = r.standard_normal((30, 10)).astype(np.float32)
+    d = r.standard_normal(30).astype(np.float32)
+    m = np.zeros(10, np.float32)
+    res = np.zeros(8)
+    ref.ref_cgstep_solve.restype = C.c_int
+    n = ref.ref_cgstep_solve(30, 10, C.c_void_p(A.ctypes.data), C.c_void_p(d.ctypes.data), 8,
+                             C.c_void_p(m.ctypes.data), C.c_void_p(res.ctypes.data))
+    assert n == 8
+    m64, res64 = O.cgstep_solve(A.astype(np.float64), d.astype(np.float64), 8)
+    np.testing.assert_allclose(m, m64, rtol=2e-4, atol=2e-5)
+    np.testing.assert_allclose(res, res64, rtol=2e-4)
+
+
+# --- geometry ------------------------------------------------------------------
+def test_source_geometry_hand_example():
+    locs, tr = O.source_geometry(10, 12, 5.0, 4.0, 6.0, np.array([1.0, 2.0]), 8)
+    # plane iz-1 = 4, columns 3..5 x rows 5..7, centre weight exp(-1) (zn = 1)
+    assert locs[4] == 4 + 6 * 10 + 4 * 120
+    assert locs[0] == 3 + 5 * 10 + 4 * 120
+    assert np.isclose(tr[4, 4], math.exp(-1.0)) and np.isclose(tr[4, 5], 2 * math.exp(-1.0))
+    assert np.isclose(tr[0, 4], math.exp(-3.0))
+    assert np.all(tr[:, :4] == 0)
+
+
+def test_receiver_index_hand_example():
+    assert O.receiver_index(10, 12, [2.7], [3.1], [5.9])[0] == 2 + 3 * 10 + (5 + 4) * 120
