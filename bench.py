@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rtm512")
     ap.add_argument("--nt", type=int, default=0, help="override nt (testing only)")
-    ap.add_argument("--ty", type=int, default=16)
+    ap.add_argument("--ty", type=int, default=0, help="tile rows (0 = per-variant default)")
     ap.add_argument("--target-blocks", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -132,7 +132,7 @@ def main():
         dev.sync()
         log(f"warmup {i + 1}/{a.warmup} done")
 
-    dev.kernel_timing(True, 0, 0)
+    dev.kernel_timing(True, 0, 0, 0)
     dev.sync()
     t1 = time.perf_counter()
     for i in range(a.steps):

@@ -143,19 +143,27 @@ int brtm_get_snapshot(brtm_ctx *ctx, int k, float *out);
  * newest and previous fields. */
 int brtm_propagate(brtm_ctx *ctx, int form, int nsteps, float *p_cur, float *p_old);
 
-/* Kernel timing: run nrep steps of the stencil (form 0/1, fmt 0..3 fused
- * imaging/Born-source format) on the context's resident fields between two
- * HIP events on the library stream; avg_ms receives the mean duration. */
-int brtm_time_step(brtm_ctx *ctx, int form, int fmt, int nrep, double *avg_ms);
+/* Kernel timing: run nrep steps of one stencil variant on the context's
+ * resident fields between two HIP events on the library stream; avg_ms
+ * receives the mean duration.  form 0 = VL, 1 = LV; act 0 = none,
+ * 1 = fused imaging (VL only), 2 = fused Born source; fmt 0..3 = snapshot
+ * format (none, FP32, BF16, FP16; none exactly when act is none). */
+int brtm_time_step(brtm_ctx *ctx, int form, int act, int fmt, int nrep, double *avg_ms);
 
-/* Per-launch timing of one stencil variant (form 0/1, fmt 0..3): while
+/* Born modelling path: 0 (default) advances u = V.w.q with the forward-form
+ * kernel (the transpose-form step equals D^-1 A_VL D, D = V.w); 1 runs the
+ * transpose-form kernel on q directly (cross-check). */
+int brtm_set_born_path(brtm_ctx *ctx, int direct_lv);
+
+/* Per-launch timing of one stencil variant (form, act, fmt as above): while
  * enabled, every launch of that variant is bracketed by HIP events on the
  * library stream; the result is the summed duration and launch count since
  * the last brtm_kernel_timing call. */
-int brtm_kernel_timing(brtm_ctx *ctx, int enable, int form, int fmt);
+int brtm_kernel_timing(brtm_ctx *ctx, int enable, int form, int act, int fmt);
 int brtm_kernel_timing_result(brtm_ctx *ctx, double *total_ms, int *count);
 
-/* Tile rows per workgroup (8, 16 or 32) and target workgroup count. */
+/* Tile rows per workgroup (8, 16 or 32; 0 = chosen per step variant, the
+ * default) and target workgroup count (used when ty_tile != 0). */
 int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 
 /* Device synchronisation of the library stream. */

@@ -61,9 +61,10 @@ SIGNATURES = {
     "brtm_set_imaging_scale": [_P, _F],
     "brtm_get_snapshot": [_P, _I, _P],
     "brtm_propagate": [_P, _I, _I, _P, _P],
-    "brtm_time_step": [_P, _I, _I, _I, C.POINTER(_D)],
+    "brtm_time_step": [_P, _I, _I, _I, _I, C.POINTER(_D)],
     "brtm_set_tuning": [_P, _I, _I],
-    "brtm_kernel_timing": [_P, _I, _I, _I],
+    "brtm_set_born_path": [_P, _I],
+    "brtm_kernel_timing": [_P, _I, _I, _I, _I],
     "brtm_kernel_timing_result": [_P, C.POINTER(_D), C.POINTER(_I)],
     "brtm_sync": [_P],
     "brtm_device_bytes": [_P, C.POINTER(_D)],

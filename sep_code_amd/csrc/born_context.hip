@@ -56,7 +56,7 @@ static int fmt_of_store(int kind) {
 
 static size_t fmt_bytes(int fmt) { return fmt == FMT_F32 ? 4 : 2; }
 
-int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float *pn,
+int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img, const float *m) {
   StepParams P{};
   P.n1 = c->n1;
@@ -74,8 +74,17 @@ int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float 
   P.sscale = sscale;
   P.m = m;
   P.cimg = c->cimg;
-  plan_step(P, c->ty_tile, c->target_blocks);
-  const bool timed = c->ktime_on && form == c->ktime_form && fmt == c->ktime_fmt;
+  // Tile rows / workgroup target per variant (measured on MI355X at 512^3):
+  // plain steps prefer tall tiles, 16-bit snapshot variants short ones.
+  int ty = c->ty_tile, tb = c->target_blocks;
+  if (ty == 0) {
+    if (act == ACT_NONE) { ty = 32; tb = 1024; }
+    else if (fmt == FMT_F32) { ty = 16; tb = 8192; }
+    else { ty = 8; tb = 8192; }
+  }
+  plan_step(P, ty, tb);
+  const bool timed =
+      c->ktime_on && form == c->ktime_form && act == c->ktime_act && fmt == c->ktime_fmt;
   if (timed) {
     if (c->ktime_used + 2 > c->ktime_ev.size()) {
       for (int k = 0; k < 256; ++k) {
@@ -86,7 +95,7 @@ int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float 
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  BRTM_HIP(launch_step(form, fmt, P, c->ty_tile, c->stream));
+  BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream));
   if (timed) {
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
     c->ktime_used += 2;
@@ -134,10 +143,10 @@ static int inject_data(brtm_ctx *c, int it, int jt, float *p, const void *snap, 
   return BRTM_OK;
 }
 
-static int extract_data(brtm_ctx *c, int it, int jt, const float *p) {
+static int extract_data(brtm_ctx *c, int it, int jt, const float *p, const float *rscale) {
   if (c->nrec == 0) return BRTM_OK;
   BRTM_HIP(launch_extract(c->nrec, c->rlocs.as<int>(), c->traces.as<float>(), c->nrec, it / jt,
-                          &c->sinc[1][8 * (it % jt)], p, c->stream));
+                          &c->sinc[1][8 * (it % jt)], p, rscale, c->stream));
   return BRTM_OK;
 }
 
@@ -193,7 +202,8 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
       BRTM_HIP(hipMemcpyAsync(base, cur, fbytes, hipMemcpyDeviceToDevice, c->stream));
       BRTM_HIP(hipMemcpyAsync(base + fbytes, old, fbytes, hipMemcpyDeviceToDevice, c->stream));
     }
-    BRTM_TRY(step(c, FORM_VL, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr, nullptr));
+    BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr,
+                  nullptr));
     BRTM_TRY(inject_source(c, it, jt, old));
     if (fmt != FMT_NONE && it % jimg == 0) {
       const int k = it / jimg;
@@ -228,6 +238,28 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
   if (!accumulate && c->nrec)
     BRTM_HIP(hipMemsetAsync(c->traces.ptr, 0, sizeof(float) * (size_t)c->nrec * c->ntr_d,
                             c->stream));
+  // The transpose-form step is a diagonal similarity of the forward form:
+  // A_LV = D^-1 A_VL D with D = V.w (and the w^2 term commutes with D).  So
+  // the scattered field q is advanced as u = D q with the VL kernel: the
+  // Born source becomes cimg.s.(D m) and receivers read u / D.  The direct
+  // LV kernel path is kept (born_direct_lv) as a cross-check.
+  const bool direct = c->born_direct_lv;
+  const float *mdl = c->mdl.as<float>();
+  const float *rscale = nullptr;
+  if (!direct) {
+    BRTM_TRY(c->mvw.alloc(fbytes));
+    BRTM_HIP(launch_vw_scale(c->V.as<float>(), c->d_wtab.as<float>(), c->nw, mdl,
+                             c->mvw.as<float>(), c->n1, c->n2, c->n3, c->stream));
+    mdl = c->mvw.as<float>();
+    if (c->nrec) {
+      BRTM_TRY(c->rscale.alloc(sizeof(float) * (size_t)c->nrec));
+      BRTM_HIP(launch_receiver_scale(c->V.as<float>(), c->d_wtab.as<float>(), c->nw,
+                                     c->rlocs.as<int>(), c->nrec, c->rscale.as<float>(), c->n1,
+                                     c->n2, c->n3, c->stream));
+      rscale = c->rscale.as<float>();
+    }
+  }
+  const int qform = direct ? FORM_LV : FORM_VL;
   float *qc = c->fd.as<float>(), *qo = c->fc.as<float>();
   float *sc = c->fb.as<float>(), *so = c->fa.as<float>();
   BRTM_HIP(hipMemsetAsync(qc, 0, fbytes, c->stream));
@@ -238,7 +270,8 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
   }
   for (int it = 0; it < nt; ++it) {
     if (live) {
-      BRTM_TRY(step(c, FORM_VL, FMT_NONE, sc, so, so, nullptr, nullptr, nullptr, nullptr));
+      BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, sc, so, so, nullptr, nullptr, nullptr,
+                    nullptr));
       BRTM_TRY(inject_source(c, it, jt, so));
     }
     if (it % jimg == 0) {
@@ -254,11 +287,12 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
         snap = S.data.as<char>() + fmt_bytes(f) * (size_t)c->ncell * k;
         if (f != FMT_F32) ss = S.scale.as<float>() + (size_t)c->n3 * k;
       }
-      BRTM_TRY(step(c, FORM_LV, f, qc, qo, qo, snap, ss, nullptr, c->mdl.as<float>()));
+      BRTM_TRY(step(c, qform, ACT_BORN, f, qc, qo, qo, snap, ss, nullptr, mdl));
     } else {
-      BRTM_TRY(step(c, FORM_LV, FMT_NONE, qc, qo, qo, nullptr, nullptr, nullptr, nullptr));
+      BRTM_TRY(step(c, qform, ACT_NONE, FMT_NONE, qc, qo, qo, nullptr, nullptr, nullptr,
+                    nullptr));
     }
-    BRTM_TRY(extract_data(c, it, jt, qo));
+    BRTM_TRY(extract_data(c, it, jt, qo, rscale));
     std::swap(qc, qo);
     if (live) std::swap(sc, so);
   }
@@ -286,10 +320,11 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
         const int k = it / jimg;
         const void *snap = S.data.as<char>() + fmt_bytes(sfmt) * (size_t)c->ncell * k;
         const float *ss = sfmt == FMT_F32 ? nullptr : S.scale.as<float>() + (size_t)c->n3 * k;
-        BRTM_TRY(step(c, FORM_VL, sfmt, lc, lo, lo, snap, ss, img, nullptr));
+        BRTM_TRY(step(c, FORM_VL, ACT_IMG, sfmt, lc, lo, lo, snap, ss, img, nullptr));
         BRTM_TRY(inject_data(c, it, jt, lo, snap, sfmt, ss, img));
       } else {
-        BRTM_TRY(step(c, FORM_VL, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr, nullptr));
+        BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr,
+                      nullptr));
         BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
       }
       std::swap(lc, lo);
@@ -312,17 +347,18 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
     BRTM_HIP(hipMemcpyAsync(slot(0), base + fbytes, fbytes, hipMemcpyDeviceToDevice, c->stream));
     for (int it = k0; it < k1; ++it) {
       const int j = it - k0;
-      BRTM_TRY(step(c, FORM_VL, FMT_NONE, slot(j + 1), slot(j), slot(j + 2), nullptr, nullptr,
-                    nullptr, nullptr));
+      BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, slot(j + 1), slot(j), slot(j + 2), nullptr,
+                    nullptr, nullptr, nullptr));
       BRTM_TRY(inject_source(c, it, jt, slot(j + 2)));
     }
     for (int it = k1 - 1; it >= k0; --it) {
       if (it % jimg == 0) {
         const float *snap = slot(it - k0 + 2);
-        BRTM_TRY(step(c, FORM_VL, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr));
+        BRTM_TRY(step(c, FORM_VL, ACT_IMG, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr));
         BRTM_TRY(inject_data(c, it, jt, lo, snap, FMT_F32, nullptr, img));
       } else {
-        BRTM_TRY(step(c, FORM_VL, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr, nullptr));
+        BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr,
+                      nullptr));
         BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
       }
       std::swap(lc, lo);
@@ -453,12 +489,20 @@ int brtm_set_velocity(brtm_ctx *c, const float *V) {
   BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
   // Leapfrog limit: max V * lambda_max(-L8) <= 4, lambda_max(-L8) = sum_a
   // C_a (1025/576 + 2 (1 + 1/8 + 1/63 + 1/896)) (symbol of L8 at pi).
+  // V must be positive on the update set: Born modelling advances
+  // u = V.w.q, which determines q only where V.w > 0.
   double vmax = 0;
   for (long i = 0; i < c->ncell; ++i) {
     const double v = V[i];
     BRTM_REQUIRE(std::isfinite(v) && v >= 0, BRTM_EINVAL, "V must be finite and >= 0");
     vmax = std::max(vmax, v);
   }
+  for (int i3 = 4; i3 < c->n3 - 4; ++i3)
+    for (int i2 = 4; i2 < c->n2 - 4; ++i2) {
+      const float *row = V + ((long)i3 * c->n2 + i2) * c->n1;
+      for (int i1 = 4; i1 < c->n1 - 4; ++i1)
+        BRTM_REQUIRE(row[i1] > 0.f, BRTM_EINVAL, "V must be > 0 on the update set");
+    }
   const double S = 1025.0 / 576.0 + 2.0 * (1.0 + 1.0 / 8 + 1.0 / 63 + 1.0 / 896);
   double lam = 0;
   for (int a = 0; a < 3; ++a) lam += 8.0 / (5.0 * (double)c->d[a] * c->d[a]) * S;
@@ -657,7 +701,8 @@ int brtm_propagate(brtm_ctx *c, int form, int nsteps, float *p_cur, float *p_old
   BRTM_HIP(hipMemcpyAsync(cur, p_cur, fbytes, hipMemcpyHostToDevice, c->stream));
   BRTM_HIP(hipMemcpyAsync(old, p_old, fbytes, hipMemcpyHostToDevice, c->stream));
   for (int s = 0; s < nsteps; ++s) {
-    BRTM_TRY(step(c, form, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr, nullptr));
+    BRTM_TRY(
+        step(c, form, ACT_NONE, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr, nullptr));
     std::swap(cur, old);
   }
   BRTM_HIP(hipMemcpyAsync(p_cur, cur, fbytes, hipMemcpyDeviceToHost, c->stream));
@@ -666,10 +711,13 @@ int brtm_propagate(brtm_ctx *c, int form, int nsteps, float *p_cur, float *p_old
   return BRTM_OK;
 }
 
-int brtm_time_step(brtm_ctx *c, int form, int fmt, int nrep, double *avg_ms) {
+int brtm_time_step(brtm_ctx *c, int form, int act, int fmt, int nrep, double *avg_ms) {
   BRTM_REQUIRE(c && avg_ms && nrep >= 1, BRTM_EINVAL, "bad arguments");
   BRTM_REQUIRE(form == 0 || form == 1, BRTM_EINVAL, "form must be 0 or 1");
+  BRTM_REQUIRE(act >= ACT_NONE && act <= ACT_BORN, BRTM_EINVAL, "act must be 0..2");
   BRTM_REQUIRE(fmt >= FMT_NONE && fmt <= FMT_F16, BRTM_EINVAL, "fmt must be 0..3");
+  BRTM_REQUIRE((act == ACT_NONE) == (fmt == FMT_NONE), BRTM_EINVAL,
+               "an action needs a snapshot format and vice versa");
   BRTM_REQUIRE(c->have_space && c->have_vel, BRTM_ESTATE, "space/velocity not set");
   BRTM_HIP(hipSetDevice(c->device));
   DevBuf sc;
@@ -677,13 +725,13 @@ int brtm_time_step(brtm_ctx *c, int form, int fmt, int nrep, double *avg_ms) {
   BRTM_HIP(launch_fill(sc.as<float>(), 1.f, c->n3, c->stream));
   float *cur = c->fb.as<float>(), *old = c->fa.as<float>();
   const void *snap = c->fd.ptr;  // any resident field serves as the snapshot
-  float *img = form == 0 && fmt ? c->img.as<float>() : nullptr;
-  const float *m = form == 1 && fmt ? c->mdl.as<float>() : nullptr;
+  float *img = act == ACT_IMG ? c->img.as<float>() : nullptr;
+  const float *m = act == ACT_BORN ? c->mdl.as<float>() : nullptr;
   // warm-up launch, then the timed run
-  BRTM_TRY(step(c, form, fmt, cur, old, old, snap, sc.as<float>(), img, m));
+  BRTM_TRY(step(c, form, act, fmt, cur, old, old, snap, sc.as<float>(), img, m));
   BRTM_HIP(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nrep; ++s) {
-    BRTM_TRY(step(c, form, fmt, cur, old, old, snap, sc.as<float>(), img, m));
+    BRTM_TRY(step(c, form, act, fmt, cur, old, old, snap, sc.as<float>(), img, m));
     std::swap(cur, old);
   }
   BRTM_HIP(hipEventRecord(c->ev1, c->stream));
@@ -697,19 +745,26 @@ int brtm_time_step(brtm_ctx *c, int form, int fmt, int nrep, double *avg_ms) {
 
 int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
-  BRTM_REQUIRE(ty_tile == 8 || ty_tile == 16 || ty_tile == 32, BRTM_EINVAL,
-               "ty_tile must be 8, 16 or 32");
+  BRTM_REQUIRE(ty_tile == 0 || ty_tile == 8 || ty_tile == 16 || ty_tile == 32, BRTM_EINVAL,
+               "ty_tile must be 0 (automatic), 8, 16 or 32");
   BRTM_REQUIRE(target_blocks >= 1, BRTM_EINVAL, "target_blocks must be >= 1");
   c->ty_tile = ty_tile;
   c->target_blocks = target_blocks;
   return BRTM_OK;
 }
 
-int brtm_kernel_timing(brtm_ctx *c, int enable, int form, int fmt) {
+int brtm_set_born_path(brtm_ctx *c, int direct_lv) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  c->born_direct_lv = direct_lv != 0;
+  return BRTM_OK;
+}
+
+int brtm_kernel_timing(brtm_ctx *c, int enable, int form, int act, int fmt) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
   BRTM_HIP(hipStreamSynchronize(c->stream));
   c->ktime_on = enable != 0;
   c->ktime_form = form;
+  c->ktime_act = act;
   c->ktime_fmt = fmt;
   c->ktime_used = 0;
   return BRTM_OK;
@@ -740,7 +795,7 @@ int brtm_device_bytes(brtm_ctx *c, double *bytes) {
   double b = 0;
   for (DevBuf *x : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
                     &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs,
-                    &c->traces})
+                    &c->traces, &c->mvw, &c->rscale})
     b += (double)x->bytes;
   for (auto &s : c->stores) b += (double)s.data.bytes + (double)s.scale.bytes;
   *bytes = b;

@@ -70,9 +70,11 @@ struct brtm_ctx {
   int nw = 0;
   std::vector<float> wtab;
   float cimg = 1e-6f;
-  int ty_tile = 16, target_blocks = 2048;
+  int ty_tile = 0, target_blocks = 2048;  // ty_tile 0: chosen per step variant
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
+  brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)
+  bool born_direct_lv = false;
   int seg_slots = 0;
   brtm::DevBuf partials, dotv;
 
@@ -95,7 +97,7 @@ struct brtm_ctx {
   // Per-launch timing of one (form, fmt) stencil variant: an event pair is
   // recorded around every such launch while enabled.
   bool ktime_on = false;
-  int ktime_form = 0, ktime_fmt = 0;
+  int ktime_form = 0, ktime_act = 0, ktime_fmt = 0;
   std::vector<hipEvent_t> ktime_ev;  // 2 per launch
   size_t ktime_used = 0;
 };
@@ -109,7 +111,7 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate);
 int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg);
 int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out);
 Store &cur_store(brtm_ctx *c);
-int step(brtm_ctx *c, int form, int fmt, const float *p, const float *po, float *pn,
+int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img_or_null, const float *m_or_null);
 
 }  // namespace brtm

@@ -10,12 +10,17 @@ namespace brtm {
 enum SnapFmt { FMT_NONE = 0, FMT_F32 = 1, FMT_BF16 = 2, FMT_F16 = 3 };
 // Step form: VL = V.L(p) (Born/wave_fkernel.3d8o.cu:31), LV = L(V.p) (:109).
 enum StepForm { FORM_VL = 0, FORM_LV = 1 };
+// What the step does with the snapshot s besides the update:
+//   ACT_IMG  img += cimg . s . p_new        (zero-lag imaging, VL only)
+//   ACT_BORN p_new += cimg . s . m          (Born secondary source)
+enum StepAct { ACT_NONE = 0, ACT_IMG = 1, ACT_BORN = 2 };
 
 constexpr int kWMax = 256;  // longest sponge table the kernels accept
 
 // One time step of the fused stencil:
-//   VL: pn = w.(V.L p + 2p) - w^2.po          [+ img += cimg.s.pn]
-//   LV: pn = L(V.w.p) + 2w.p - w^2.po          [+ pn  += cimg.s.m ]
+//   VL: pn = w.(V.L p + 2p) - w^2.po
+//   LV: pn = L(V.w.p) + 2w.p - w^2.po
+// followed by the action (StepAct).
 // s is the source snapshot (format fmt, per-slice scale sscale for 16-bit).
 struct StepParams {
   int n1, n2, n3, nw;
@@ -33,7 +38,7 @@ struct StepParams {
   float cimg;
 };
 
-hipError_t launch_step(int form, int fmt, StepParams P, int ty_tile, hipStream_t st);
+hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // Tile / chunk geometry chosen for a grid (fills ntx..zc of P).
 void plan_step(StepParams &P, int ty_tile, int target_blocks);
 
@@ -58,8 +63,17 @@ struct InjectParams {
 hipError_t launch_inject(const InjectParams &P, hipStream_t st);
 
 // tr(j, id+k) += sinc8[k] * p[loc_j]  (device trace layout [t][rec]).
+// With rscale != null the extracted value is p[loc_j] * rscale[j].
 hipError_t launch_extract(int npts, const int *locs, float *tr, long st_t, int id,
-                          const float sinc8[8], const float *p, hipStream_t st);
+                          const float sinc8[8], const float *p, const float *rscale,
+                          hipStream_t st);
+// out[i] = V[i] * w(i) * in[i] over the compute grid (w the sponge weight,
+// 0 outside the update set); the diagonal D with A_LV = D^-1 A_VL D.
+hipError_t launch_vw_scale(const float *V, const float *wtab, int nw, const float *in,
+                           float *out, int n1, int n2, int n3, hipStream_t st);
+// rs[j] = 1 / (V w)[loc_j] (0 for inactive receivers).
+hipError_t launch_receiver_scale(const float *V, const float *wtab, int nw, const int *locs,
+                                 int npts, float *rs, int n1, int n2, int n3, hipStream_t st);
 
 // Snapshot encoding: per-slice max |s| then quantise to fmt.
 hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int n12, int n3,

@@ -64,9 +64,10 @@ __device__ __forceinline__ int edist(int i, int n) { return min(i - 4, n - 4 - i
 // ---------------------------------------------------------------------------
 // The fused stencil step.
 // ---------------------------------------------------------------------------
-template <int FORM, int TY, int RPT, int FMT>
+template <int FORM, int TY, int RPT, int FMT, int ACT>
 __global__ void __launch_bounds__(64 * (TY / RPT))
 k_step(const StepParams P) {
+  static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
   constexpr int NWV = TY / RPT;  // waves per workgroup
   constexpr int HY = 8 / NWV;    // y-halo rows loaded per wave
   static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
@@ -168,9 +169,9 @@ k_step(const StepParams P) {
       const int i = z * s3 + y * n1 + x;
       opo[r] = ok ? P.po[i] : 0.f;
       if constexpr (FORM == FORM_VL) oV[r] = ok ? P.V[i] : 0.f;
-      if constexpr (FMT != FMT_NONE) {
+      if constexpr (ACT != ACT_NONE) {
         oS[r] = ok ? snap_at<FMT>(P.snap, P.sscale, i, z) : 0.f;
-        oX[r] = ok ? (FORM == FORM_VL ? P.img[i] : P.m[i]) : 0.f;
+        oX[r] = ok ? (ACT == ACT_IMG ? P.img[i] : P.m[i]) : 0.f;
       }
     }
   };
@@ -227,13 +228,15 @@ k_step(const StepParams P) {
         float pn;
         if constexpr (FORM == FORM_VL) {
           pn = w * fmaf(cV[r], acc, 2.f * cc) - (w * w) * cpo[r];
-          P.pn[i] = pn;
-          if constexpr (FMT != FMT_NONE) P.img[i] = fmaf(P.cimg * cS[r], pn, cX[r]);
         } else {
           pn = acc + 2.f * w * rq[r][0] - (w * w) * cpo[r];
-          if constexpr (FMT != FMT_NONE) pn = fmaf(P.cimg * cS[r], cX[r], pn);
-          P.pn[i] = pn;
         }
+        if constexpr (ACT == ACT_BORN) pn = fmaf(P.cimg * cS[r], cX[r], pn);
+        // The new field and the image are not read again in this launch:
+        // stream them past the caches.
+        __builtin_nontemporal_store(pn, P.pn + i);
+        if constexpr (ACT == ACT_IMG)
+          __builtin_nontemporal_store(fmaf(P.cimg * cS[r], pn, cX[r]), P.img + i);
       }
     }
 
@@ -283,17 +286,50 @@ __global__ void k_inject(const InjectParams P) {
 
 __global__ void k_extract(int npts, const int *__restrict__ locs, float *tr, long st_t, int id,
                           float s0, float s1, float s2, float s3, float s4, float s5, float s6,
-                          float s7, const float *__restrict__ p) {
+                          float s7, const float *__restrict__ p, const float *__restrict__ rs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npts) return;
   const int loc = locs[j];
   if (loc < 0) return;
-  const float v = p[loc];
+  const float v = rs ? p[loc] * rs[j] : p[loc];
   const float s[8] = {s0, s1, s2, s3, s4, s5, s6, s7};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if (s[k] != 0.f) tr[(long)(id + k) * st_t + j] += v * s[k];
   }
+}
+
+// Sponge weight of a cell, 0 outside the update set.
+__device__ __forceinline__ float cell_weight(const float *wtab, int nw, int i1, int i2, int i3,
+                                             int n1, int n2, int n3) {
+  const int e = min(min(edist(i1, n1), edist(i2, n2)), edist(i3, n3));
+  if (e < 0) return 0.f;
+  return e < nw ? wtab[e] : 1.f;
+}
+
+__global__ void k_vw_scale(const float *__restrict__ V, const float *__restrict__ wtab, int nw,
+                           const float *__restrict__ in, float *out, int n1, int n2, int n3) {
+  const long n = (long)n1 * n2 * n3;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int i1 = (int)(i % n1), i2 = (int)((i / n1) % n2), i3 = (int)(i / ((long)n1 * n2));
+    out[i] = V[i] * cell_weight(wtab, nw, i1, i2, i3, n1, n2, n3) * in[i];
+  }
+}
+
+__global__ void k_receiver_scale(const float *__restrict__ V, const float *__restrict__ wtab,
+                                 int nw, const int *__restrict__ locs, int npts, float *rs, int n1,
+                                 int n2, int n3) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  const int loc = locs[j];
+  float v = 0.f;
+  if (loc >= 0) {
+    const int i1 = loc % n1, i2 = (loc / n1) % n2, i3 = loc / (n1 * n2);
+    const float d = V[loc] * cell_weight(wtab, nw, i1, i2, i3, n1, n2, n3);
+    v = d > 0.f ? 1.f / d : 0.f;
+  }
+  rs[j] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -321,7 +357,9 @@ __global__ void k_encode(const float *__restrict__ s, void *dst, const float *__
                          int n12) {
   const int z = blockIdx.y;
   const float sc = sscale[z];
-  const float inv = sc > 0.f ? 1.f / sc : 0.f;
+  // A slice whose maximum is below 1e-30 (including denormals, whose
+  // reciprocal overflows) is stored as zeros.
+  const float inv = sc >= 1e-30f ? 1.f / sc : 0.f;
   const long base = (long)z * n12;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += gridDim.x * blockDim.x) {
     const float v = s[base + i] * inv;
@@ -505,18 +543,32 @@ int grid_for(long n, int bs) {
   return (int)g;
 }
 
-template <int FORM, int TY, int RPT>
-hipError_t launch_step_t(int fmt, const StepParams &P, hipStream_t st) {
+template <int FORM, int TY, int RPT, int ACT>
+hipError_t launch_step_a(int fmt, const StepParams &P, hipStream_t st) {
   const dim3 block(64, TY / RPT);
-  const dim3 grid(P.ntx * P.nty * P.ntz);
+  const dim3 grid(P.ntx * P.ntz * P.nty);
   switch (fmt) {
-    case FMT_NONE: k_step<FORM, TY, RPT, FMT_NONE><<<grid, block, 0, st>>>(P); break;
-    case FMT_F32: k_step<FORM, TY, RPT, FMT_F32><<<grid, block, 0, st>>>(P); break;
-    case FMT_BF16: k_step<FORM, TY, RPT, FMT_BF16><<<grid, block, 0, st>>>(P); break;
-    case FMT_F16: k_step<FORM, TY, RPT, FMT_F16><<<grid, block, 0, st>>>(P); break;
+    case FMT_F32: k_step<FORM, TY, RPT, FMT_F32, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_BF16: k_step<FORM, TY, RPT, FMT_BF16, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_F16: k_step<FORM, TY, RPT, FMT_F16, ACT><<<grid, block, 0, st>>>(P); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int FORM, int TY, int RPT>
+hipError_t launch_step_t(int act, int fmt, const StepParams &P, hipStream_t st) {
+  if (act == ACT_NONE) {
+    if (fmt != FMT_NONE) return hipErrorInvalidValue;
+    k_step<FORM, TY, RPT, FMT_NONE, ACT_NONE><<<dim3(P.ntx * P.nty * P.ntz), dim3(64, TY / RPT),
+                                                0, st>>>(P);
+    return hipGetLastError();
+  }
+  if (act == ACT_BORN) return launch_step_a<FORM, TY, RPT, ACT_BORN>(fmt, P, st);
+  if constexpr (FORM == FORM_VL) {
+    if (act == ACT_IMG) return launch_step_a<FORM, TY, RPT, ACT_IMG>(fmt, P, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -533,16 +585,16 @@ void plan_step(StepParams &P, int ty_tile, int target_blocks) {
   P.ntz = (nz + P.zc - 1) / P.zc;
 }
 
-hipError_t launch_step(int form, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {
   if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 9) return hipErrorInvalidValue;
   if (form == FORM_VL) {
-    if (ty_tile == 32) return launch_step_t<FORM_VL, 32, 4>(fmt, P, st);
-    if (ty_tile == 8) return launch_step_t<FORM_VL, 8, 2>(fmt, P, st);
-    return launch_step_t<FORM_VL, 16, 4>(fmt, P, st);
+    if (ty_tile == 32) return launch_step_t<FORM_VL, 32, 4>(act, fmt, P, st);
+    if (ty_tile == 8) return launch_step_t<FORM_VL, 8, 2>(act, fmt, P, st);
+    return launch_step_t<FORM_VL, 16, 4>(act, fmt, P, st);
   }
-  if (ty_tile == 32) return launch_step_t<FORM_LV, 32, 4>(fmt, P, st);
-  if (ty_tile == 8) return launch_step_t<FORM_LV, 8, 2>(fmt, P, st);
-  return launch_step_t<FORM_LV, 16, 4>(fmt, P, st);
+  if (ty_tile == 32) return launch_step_t<FORM_LV, 32, 4>(act, fmt, P, st);
+  if (ty_tile == 8) return launch_step_t<FORM_LV, 8, 2>(act, fmt, P, st);
+  return launch_step_t<FORM_LV, 16, 4>(act, fmt, P, st);
 }
 
 hipError_t launch_inject(const InjectParams &P, hipStream_t st) {
@@ -552,10 +604,24 @@ hipError_t launch_inject(const InjectParams &P, hipStream_t st) {
 }
 
 hipError_t launch_extract(int npts, const int *locs, float *tr, long st_t, int id,
-                          const float s[8], const float *p, hipStream_t st) {
+                          const float s[8], const float *p, const float *rscale,
+                          hipStream_t st) {
   if (npts <= 0) return hipSuccess;
   k_extract<<<(npts + 255) / 256, 256, 0, st>>>(npts, locs, tr, st_t, id, s[0], s[1], s[2], s[3],
-                                                s[4], s[5], s[6], s[7], p);
+                                                s[4], s[5], s[6], s[7], p, rscale);
+  return hipGetLastError();
+}
+
+hipError_t launch_vw_scale(const float *V, const float *wtab, int nw, const float *in, float *out,
+                           int n1, int n2, int n3, hipStream_t st) {
+  k_vw_scale<<<grid_for((long)n1 * n2 * n3, 256), 256, 0, st>>>(V, wtab, nw, in, out, n1, n2, n3);
+  return hipGetLastError();
+}
+
+hipError_t launch_receiver_scale(const float *V, const float *wtab, int nw, const int *locs,
+                                 int npts, float *rs, int n1, int n2, int n3, hipStream_t st) {
+  if (npts <= 0) return hipSuccess;
+  k_receiver_scale<<<(npts + 255) / 256, 256, 0, st>>>(V, wtab, nw, locs, npts, rs, n1, n2, n3);
   return hipGetLastError();
 }
 

@@ -142,13 +142,16 @@ class BornDevice:
         call("brtm_propagate", self.h, int(form), int(nsteps), ptr(a), ptr(b))
         return a, b
 
-    def time_step(self, form, fmt, nrep):
+    def time_step(self, form, act, fmt, nrep):
         ms = C.c_double()
-        call("brtm_time_step", self.h, int(form), int(fmt), int(nrep), C.byref(ms))
+        call("brtm_time_step", self.h, int(form), int(act), int(fmt), int(nrep), C.byref(ms))
         return ms.value
 
-    def kernel_timing(self, enable, form=0, fmt=0):
-        call("brtm_kernel_timing", self.h, int(bool(enable)), int(form), int(fmt))
+    def set_born_path(self, direct_lv):
+        call("brtm_set_born_path", self.h, int(bool(direct_lv)))
+
+    def kernel_timing(self, enable, form=0, act=0, fmt=0):
+        call("brtm_kernel_timing", self.h, int(bool(enable)), int(form), int(act), int(fmt))
 
     def kernel_timing_result(self):
         t = C.c_double()
@@ -156,7 +159,7 @@ class BornDevice:
         call("brtm_kernel_timing_result", self.h, C.byref(t), C.byref(n))
         return t.value, n.value
 
-    def set_tuning(self, ty_tile=16, target_blocks=2048):
+    def set_tuning(self, ty_tile=0, target_blocks=2048):
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
     def sync(self):

@@ -64,7 +64,7 @@ def test_step_parity(dev, n, form, ty):
     U = np.zeros(g_cur.shape, bool)
     U[4:-4, 4:-4, 4:-4] = True
     assert np.all(g_cur[~U] == 0)
-    dev.set_tuning(16, 2048)
+    dev.set_tuning(0, 2048)
 
 
 def make_shot(dev, n=(48, 44, 40), nt=120, jt=1, jimg=1, seed=3, cimg=1e-3, bc_a=8.0,
@@ -108,18 +108,42 @@ def test_source_prop_parity(dev, jt, jimg):
         assert rel(dev.get_snapshot(k), snaps[k]) < 1e-5, k
 
 
+@pytest.mark.parametrize("direct", [False, True])
 @pytest.mark.parametrize("store", ["f32", "none", "ckpt"])
-def test_born_forward_parity(dev, store):
-    announce(f"born_forward parity store={store}")
+def test_born_forward_parity(dev, store, direct):
+    """Born modelling against the oracle's transpose-form recursion, both on
+    the default path (forward-form kernel on u = V.w.q) and on the direct
+    transpose-form kernel."""
+    announce(f"born_forward parity store={store} direct_lv={direct}")
     P, r = make_shot(dev, jimg=2)
+    dev.set_born_path(direct)
     dev.source_prop(1, P.nt, 2, store=store, ckpt=17)
     m = random_field(r, P.n)
     dev.set_model(m)
     dev.rtm_forward(1, P.nt, 2)
+    dev.set_born_path(False)
     D = dev.get_receivers()
     snaps, _ = P.source_prop()
     Do = P.born_forward(m.ravel(), snaps)
     assert rel(D, Do) < 1e-5
+
+
+@pytest.mark.parametrize("store,bits", [("bf16", 8), ("f16", 11)])
+def test_snapshot_codec_roundtrip(dev, store, bits):
+    """Per-z-slice scaled 16-bit snapshots: every value within half an ulp of
+    its slice maximum (2^-(bits+1) relative), zero slices stay zero."""
+    announce(f"codec roundtrip {store}")
+    P, r = make_shot(dev, nt=60)
+    dev.source_prop(1, P.nt, 1, store="f32")
+    ref = [dev.get_snapshot(k) for k in (0, 10, 30, 59)]
+    dev.source_prop(1, P.nt, 1, store=store)
+    for k, s32 in zip((0, 10, 30, 59), ref):
+        s16 = dev.get_snapshot(k)
+        assert np.isfinite(s16).all()
+        smax = np.abs(s32).max(axis=(1, 2), keepdims=True)
+        err = np.abs(s16 - s32)
+        assert np.all(err <= smax * 2.0 ** -(bits + 1) * 1.01 + 1e-30), k
+        assert np.all(s16[smax[:, 0, 0] == 0] == 0)
 
 
 @pytest.mark.parametrize("store", ["f32", "ckpt"])
