@@ -385,3 +385,68 @@ def wavelet_derivative(w, dt):
     out = np.zeros_like(w)
     out[:-1] = (w[1:] - w[:-1]) / dt
     return out
+
+
+def mask_inactive(locs, n):
+    """Receivers outside the update set read a field that is always zero:
+    mark them -1 (they then neither extract nor inject)."""
+    n1, n2, n3 = n
+    rl = np.asarray(locs, np.int64).copy()
+    i1, i2, i3 = rl % n1, (rl // n1) % n2, rl // (n1 * n2)
+    inside = (i1 >= 4) & (i1 < n1 - 4) & (i2 >= 4) & (i2 < n2 - 4) & (i3 >= 4) & (i3 < n3 - 4)
+    rl[~inside] = -1
+    return rl.astype(np.int32)
+
+
+class CompositeBorn:
+    """FP64 restatement of rtm_zero_op::forward / adjoint
+    (Born/rtm_zero_op_3d.cpp:87-293) over all shots:
+    L m = D_t W B D_z E (mute . m), data [shot][rec][nt_data]."""
+
+    def __init__(self, n, d, V, bc_a, bc_b, shots, nt_data, dt_data, jt=1, jimg=1, cimg=1e-6,
+                 model_box=None, mute=None):
+        self.n = tuple(n)
+        n1, n2, n3 = n
+        self.box = model_box or (n1, n2, n3, 0, 0, 0)
+        self.mute = None if mute is None else np.asarray(mute, np.float64)
+        self.nt_data, self.dt = nt_data, dt_data
+        nt = (nt_data - 1) * jt + 1
+        self.ntr = nt_data + 8
+        self.probs, self.snaps = [], []
+        for (sl, st, rl) in shots:
+            P = Problem(n, d, V, bc_a, bc_b, src_locs=sl, src_tr=np.asarray(st, np.float64),
+                        rec_locs=mask_inactive(rl, n), ntr_d=self.ntr, jt=jt, nt=nt, jimg=jimg,
+                        cimg=cimg)
+            self.probs.append(P)
+            self.snaps.append(P.source_prop()[0])
+
+    def _embed(self, m):
+        m1, m2, m3, o1, o2, o3 = self.box
+        n1, n2, n3 = self.n
+        m = np.asarray(m, np.float64).reshape(m3, m2, m1)
+        if self.mute is not None:
+            m = m * self.mute.reshape(m3, m2, m1)
+        comp = np.zeros((n3, n2, n1))
+        comp[o3:o3 + m3, o2:o2 + m2, o1:o1 + m1] = m
+        return comp
+
+    def forward(self, m):
+        mt = dz_forward(self._embed(m)).ravel()
+        out = []
+        for P, S in zip(self.probs, self.snaps):
+            D = P.born_forward(mt, S)
+            out.append(dt_forward(window_forward(D, self.nt_data), self.dt))
+        return np.stack(out)
+
+    def adjoint(self, data):
+        n1, n2, n3 = self.n
+        img = np.zeros(n1 * n2 * n3)
+        for s, (P, S) in enumerate(zip(self.probs, self.snaps)):
+            tr = window_adjoint(dt_adjoint(data[s], self.dt), self.ntr)
+            img += P.rtm_adjoint(tr, S)
+        tmp = dz_adjoint(img.reshape(n3, n2, n1))
+        m1, m2, m3, o1, o2, o3 = self.box
+        m = tmp[o3:o3 + m3, o2:o2 + m2, o1:o1 + m1].copy()
+        if self.mute is not None:
+            m *= self.mute.reshape(m3, m2, m1)
+        return m

@@ -56,6 +56,17 @@ static int fmt_of_store(int kind) {
 
 static size_t fmt_bytes(int fmt) { return fmt == FMT_F32 ? 4 : 2; }
 
+// Host-to-device copy ordered on the library stream: work already queued
+// may still read the destination (the stream is non-blocking, so a plain
+// hipMemcpy on the null stream would not wait for it), and the host buffer
+// is pageable, so the copy completes before returning.
+static hipError_t h2d(brtm_ctx *c, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img, const float *m) {
   StepParams P{};
@@ -437,6 +448,7 @@ int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, flo
   BRTM_REQUIRE(bc_a >= 0 && bc_a <= (float)kWMax && bc_b >= 0, BRTM_EINVAL,
                "sponge parameters out of range (0 <= bc_a <= 256, bc_b >= 0)");
   BRTM_HIP(hipSetDevice(c->device));
+  BRTM_HIP(hipStreamSynchronize(c->stream));  // buffers below are reallocated
   c->n1 = n1;
   c->n2 = n2;
   c->n3 = n3;
@@ -462,8 +474,7 @@ int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, flo
   }
   const size_t fbytes = sizeof(float) * (size_t)c->ncell;
   BRTM_TRY(c->d_wtab.alloc(sizeof(float) * c->wtab.size()));
-  BRTM_HIP(hipMemcpy(c->d_wtab.ptr, c->wtab.data(), sizeof(float) * c->wtab.size(),
-                     hipMemcpyHostToDevice));
+  BRTM_HIP(h2d(c, c->d_wtab.ptr, c->wtab.data(), sizeof(float) * c->wtab.size()));
   for (DevBuf *b : {&c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl}) {
     b->release();
     BRTM_TRY(b->alloc(fbytes));
@@ -510,7 +521,7 @@ int brtm_set_velocity(brtm_ctx *c, const float *V) {
                "velocity violates the 3-D 8th-order leapfrog stability limit "
                "(max v dt / d above 0.4529 for equal spacing)");
   BRTM_HIP(hipSetDevice(c->device));
-  BRTM_HIP(hipMemcpy(c->V.ptr, V, sizeof(float) * (size_t)c->ncell, hipMemcpyHostToDevice));
+  BRTM_HIP(h2d(c, c->V.ptr, V, sizeof(float) * (size_t)c->ncell));
   c->have_vel = true;
   return BRTM_OK;
 }
@@ -540,14 +551,15 @@ int brtm_set_source(brtm_ctx *c, int npts, int ntr, const int *locs, const float
     BRTM_REQUIRE(in_update_set(c, locs[j]), BRTM_EINVAL,
                  "source point " + std::to_string(j) + " outside the update set");
   BRTM_HIP(hipSetDevice(c->device));
+  // queued work may still read the current source buffers
+  BRTM_HIP(hipStreamSynchronize(c->stream));
   c->nsrc = npts;
   c->ntr_s = ntr;
   if (npts == 0) return BRTM_OK;
   BRTM_TRY(c->slocs.alloc(sizeof(int) * npts));
   BRTM_TRY(c->strc.alloc(sizeof(float) * (size_t)npts * ntr));
-  BRTM_HIP(hipMemcpy(c->slocs.ptr, locs, sizeof(int) * npts, hipMemcpyHostToDevice));
-  BRTM_HIP(hipMemcpy(c->strc.ptr, vals, sizeof(float) * (size_t)npts * ntr,
-                     hipMemcpyHostToDevice));
+  BRTM_HIP(h2d(c, c->slocs.ptr, locs, sizeof(int) * npts));
+  BRTM_HIP(h2d(c, c->strc.ptr, vals, sizeof(float) * (size_t)npts * ntr));
   return BRTM_OK;
 }
 
@@ -557,6 +569,8 @@ int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, co
   BRTM_REQUIRE(nx >= 0 && ny >= 0 && ntr >= 8 && (nx * ny == 0 || locs), BRTM_EINVAL,
                "bad receiver arguments");
   BRTM_HIP(hipSetDevice(c->device));
+  // queued work may still read the current receiver buffers
+  BRTM_HIP(hipStreamSynchronize(c->stream));
   const int nrec = nx * ny;
   c->nrx = nx;
   c->nry = ny;
@@ -569,7 +583,7 @@ int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, co
   for (auto &l : c->h_rlocs)
     if (!in_update_set(c, l)) l = -1;
   BRTM_TRY(c->rlocs.alloc(sizeof(int) * nrec));
-  BRTM_HIP(hipMemcpy(c->rlocs.ptr, c->h_rlocs.data(), sizeof(int) * nrec, hipMemcpyHostToDevice));
+  BRTM_HIP(h2d(c, c->rlocs.ptr, c->h_rlocs.data(), sizeof(int) * nrec));
   const size_t tb = sizeof(float) * (size_t)nrec * ntr;
   BRTM_TRY(c->traces.alloc(tb));
   if (!vals) {
@@ -577,7 +591,7 @@ int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, co
   } else {
     DevBuf tmp;
     BRTM_TRY(tmp.alloc(tb));
-    BRTM_HIP(hipMemcpy(tmp.ptr, vals, tb, hipMemcpyHostToDevice));
+    BRTM_HIP(h2d(c, tmp.ptr, vals, tb));
     BRTM_HIP(launch_transpose(tmp.as<float>(), c->traces.as<float>(), nrec, ntr, c->stream));
     BRTM_HIP(hipStreamSynchronize(c->stream));
     tmp.release();
@@ -604,7 +618,7 @@ int brtm_set_model(brtm_ctx *c, const float *m) {
   BRTM_REQUIRE(c && m, BRTM_EINVAL, "NULL argument");
   BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
   BRTM_HIP(hipSetDevice(c->device));
-  BRTM_HIP(hipMemcpy(c->mdl.ptr, m, sizeof(float) * (size_t)c->ncell, hipMemcpyHostToDevice));
+  BRTM_HIP(h2d(c, c->mdl.ptr, m, sizeof(float) * (size_t)c->ncell));
   return BRTM_OK;
 }
 

@@ -8,6 +8,7 @@ arrays in and out.  Volumes are numpy arrays of shape (n3, n2, n1).
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -26,10 +27,18 @@ class BornDevice:
         call("brtm_create", device, C.byref(h))
         self.h = h
         self.n = None
+        self._children = weakref.WeakSet()
 
     # -- lifetime -----------------------------------------------------------
+    def adopt(self, obj):
+        """Register an object (vector, operator, network) created on this
+        context; it is closed before the context is destroyed."""
+        self._children.add(obj)
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
+            for ch in list(self._children):
+                ch.close()
             call("brtm_destroy", self.h)
             self.h = None
 
