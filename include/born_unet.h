@@ -1,0 +1,66 @@
+/*
+ * born_unet.h -- learned image-domain preconditioner of the MI355X Born
+ * RTM library.  It has no counterpart in the reference, whose inversion
+ * layer only offers an operator-chain hook (prec_solver,
+ * Born/lin_solver.cpp:21-37) and plain cgstep.
+ *
+ * Network: a 3-level 3-D U-Net on P^3 patches (P a multiple of 64), 3x3x3
+ * convolutions with 16 / 32 / 64 channels, ReLU, 2x2x2 average pooling,
+ * nearest upsampling with skip concatenation, a 1x1x1 head and a residual
+ * connection (out = x + head(...), so an untrained network is close to the
+ * identity).  Convolutions are implicit GEMMs in BF16 on MFMA with FP32
+ * accumulation; master weights, gradients and Adam state are FP32.
+ *
+ * As a preconditioner the network maps a model-space gradient g to
+ * z = s * net(g / s), s = RMS(g), applied on P^3 patches that overlap by
+ * `overlap` cells on each side (the central (P - 2 overlap)^3 of each patch
+ * is kept).  Training pairs are (RTM image, LSRTM image) made by this
+ * library's own operator and CG.
+ */
+#ifndef BORN_UNET_H
+#define BORN_UNET_H
+
+#include "born_op.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct brtm_unet brtm_unet;
+
+/* batch: patches per network pass; patch: P (multiple of 64). */
+int brtm_unet_create(brtm_ctx *ctx, int batch, int patch, unsigned long long seed,
+                     brtm_unet **out);
+int brtm_unet_destroy(brtm_unet *net);
+long brtm_unet_num_params(const brtm_unet *net);
+int brtm_unet_get_params(brtm_unet *net, float *host);
+int brtm_unet_set_params(brtm_unet *net, const float *host);
+
+/* y = net(x) for nb <= batch patches; x, y are device FP32 [nb][P^3]. */
+int brtm_unet_forward(brtm_unet *net, const float *x, float *y, int nb);
+/* loss = mean((net(x) - t)^2); gradients of the loss w.r.t. the FP32
+ * parameters into grad (device or host pointer by grad_on_host). */
+int brtm_unet_loss_grad(brtm_unet *net, const float *x, const float *t, int nb, double *loss,
+                        float *grad, int grad_on_host);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the MSE loss. */
+int brtm_unet_train_step(brtm_unet *net, const float *x, const float *t, int nb, float lr,
+                         double *loss);
+
+/* Copy the P^3 patch of a volume [m3][m2][m1] (device FP32) with origin
+ * (o1,o2,o3) (may lie partly outside: zeros) into dst (device FP32 P^3),
+ * scaled by scale. */
+int brtm_unet_extract(brtm_unet *net, const float *vol, int m1, int m2, int m3, int o1, int o2,
+                      int o3, float scale, float *dst);
+
+/* z = s net(g / s) on overlapping patches of the model grid. */
+int brtm_unet_apply(brtm_unet *net, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3,
+                    int overlap);
+/* Model grid and overlap used by brtm_unet_precond. */
+int brtm_unet_set_grid(brtm_unet *net, int m1, int m2, int m3, int overlap);
+/* brtm_precond_fn for brtm_cg_solve: user = brtm_unet*. */
+int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BORN_UNET_H */

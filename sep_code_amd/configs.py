@@ -98,6 +98,7 @@ class Workload:
     store: str = "ckpt"
     ckpt: int = 50
     cimg: float = 1e-6
+    mute_top: int = 0        # model cells below the model top that are muted
 
     @property
     def V(self):
@@ -184,10 +185,11 @@ def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_
     n = (n_side, n_side, n_side)
     dd, dt, sponge = (d, d, d), 0.001, max(8, n_side // 12)
     nbound = sponge + 4
+    shallow = max(4, n_side // 16)
     rng = SplitMix64(seed)
     n1, n2, n3 = n
     nif = rng.integers(4, 7)
-    top, bot = nbound + 8, n3 - nbound - 8
+    top, bot = nbound + shallow + 4, n3 - nbound - 8
     z0s = sorted(rng.uniform(top, bot) for _ in range(nif))
     vels = sorted(rng.uniform(1500.0, 3500.0) for _ in range(nif + 1))
     dips = [math.radians(rng.uniform(2.0, 10.0)) for _ in range(nif)]
@@ -210,8 +212,9 @@ def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_
     s = rng.uniform_array(n1 * n2 * n3).reshape(n3, n2, n1)
     sparse = (u < 0.002).astype(np.float32) * np.where(s < 0.5, -0.05, 0.05).astype(np.float32)
     refl += sparse
-    # model is zero in the sponge and the halo
-    refl[:nbound + 2] = 0
+    # model is zero in the sponge, the halo and the shallow near field of the
+    # sources and receivers (which would otherwise dominate the data)
+    refl[:nbound + shallow] = 0
     refl[n3 - nbound:] = 0
     refl[:, :nbound] = 0
     refl[:, n2 - nbound:] = 0
@@ -227,8 +230,10 @@ def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_
             sx = span1 * (ix + 0.5) / nsx
             sy = span2 * (iy + 0.5) / nsy
             shots.append(_point_source_shot(n, nbound, w, ntr, sx, sy, 2.0, rec_step=rec_step))
-    return Workload(f"lsrtm{n_side}", n, dd, dt, nt, 1, jimg, float(sponge), 0.002, nbound, vel,
-                    refl, shots, ntr, store=store)
+    wl = Workload(f"lsrtm{n_side}", n, dd, dt, nt, 1, jimg, float(sponge), 0.002, nbound, vel,
+                  refl, shots, ntr, store=store)
+    wl.mute_top = shallow  # model-grid cells muted below the model top
+    return wl
 
 
 def smoke(seed: int = 5) -> Workload:

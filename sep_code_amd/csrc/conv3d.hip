@@ -1,0 +1,596 @@
+// 3-D convolutions of the learned preconditioner on gfx950 MFMA.
+//
+// Forward (and input-gradient) convolutions are implicit GEMMs: M = output
+// channels, N = 16 consecutive x positions per wave, K = 27 taps x input
+// channels in chunks of 32, one v_mfma_f32_16x16x32_bf16 per (16-channel
+// output tile, K chunk).  Operand lanes: lane l holds A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16] (i < 8), and receives D[4(l/16)+j][l%16] (j < 4)
+// (checked on the device against a host product).  In channels-last layout
+// the 8 K values of a lane are 8 consecutive channels of one position: one
+// 16-byte load.  The weight gradient is a GEMM over positions (K) staged
+// through LDS transposed so both operands are again 16-byte lane loads.
+#include "conv3d.h"
+
+namespace brtm {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p) {
+  const uint4 u = *reinterpret_cast<const uint4 *>(p);
+  return __builtin_bit_cast(bf16x8, u);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+__host__ __device__ inline int cpad(int c) { return (c + 31) / 32 * 32; }
+int grid1(long n, int bs = 256) {
+  long g = (n + bs - 1) / bs;
+  return (int)(g < 1 ? 1 : (g > 65535 * 4 ? 65535 * 4 : g));
+}
+
+// ------------------------------------------------------------- forward GEMM
+template <int CIN_P, int COUT, bool RELU, bool OUTF32>
+__global__ void __launch_bounds__(256)
+k_conv_fwd(const uint16_t *__restrict__ x, int xcs, int cin, const uint16_t *__restrict__ W,
+           const float *__restrict__ bias, void *y, int ycs, int N, int D, int H, int Wd) {
+  constexpr int MT = COUT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wt = Wd >> 4;
+  const long tile = (long)blockIdx.x * 4 + wave;
+  if (tile >= (long)N * D * H * wt) return;  // whole wave; no barriers below
+  const int xt = (int)(tile % wt);
+  long r = tile / wt;
+  const int yy = (int)(r % H);
+  r /= H;
+  const int zz = (int)(r % D);
+  const int nn = (int)(r / D);
+  const int x0 = xt * 16;
+  const int col = lane & 15, kq = lane >> 4;
+  f32x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; ++tap) {
+    const int z2 = zz + tap / 9 - 1, y2 = yy + (tap / 3) % 3 - 1, x2 = x0 + col + tap % 3 - 1;
+    const bool ok = z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd;
+    const long pos = (((long)nn * D + z2) * H + y2) * Wd + x2;
+    const uint16_t *wtap = W + (long)tap * COUT * CIN_P;
+#pragma unroll
+    for (int c0 = 0; c0 < CIN_P; c0 += 32) {
+      const int c = c0 + 8 * kq;
+      const bf16x8 b = (ok && c < cin) ? ld8(x + pos * xcs + c) : zero8();
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const bf16x8 a = ld8(wtap + (mt * 16 + col) * CIN_P + c);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
+      }
+    }
+  }
+  const long opos = (((long)nn * D + zz) * H + yy) * Wd + x0 + col;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = mt * 16 + 4 * kq + j;
+      float v = acc[mt][j];
+      if constexpr (OUTF32) {
+        static_cast<float *>(y)[opos * ycs + co] = v;
+      } else {
+        v += bias[co];
+        if (RELU) v = fmaxf(v, 0.f);
+        static_cast<uint16_t *>(y)[opos * ycs + co] = f2bf(v);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- weight GEMM
+template <int CIN_P, int COUT>
+__global__ void __launch_bounds__(256)
+k_conv_wgrad(const uint16_t *__restrict__ x, int xcs, int cin, const uint16_t *__restrict__ dy,
+             int dycs, float *dW, int N, int D, int H, int Wd, long chunk) {
+  constexpr int MT = COUT / 16, NTC = CIN_P / 16, T = MT * NTC, TPW = (T + 3) / 4;
+  constexpr int LP = 40;  // 32 positions + pad, rows stay 16-byte aligned
+  __shared__ __attribute__((aligned(16))) uint16_t As[COUT][LP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[CIN_P][LP];
+  const int tap = blockIdx.y;
+  const int dz = tap / 9 - 1, dyy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const long npos = (long)N * D * H * Wd;
+  const long p0 = (long)blockIdx.x * chunk;
+  const long p1 = p0 + chunk < npos ? p0 + chunk : npos;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long p = p0; p < p1; p += 32) {
+    for (int i = tid; i < 32 * COUT; i += 256) {
+      const int pp = i / COUT, co = i % COUT;
+      const long q = p + pp;
+      As[co][pp] = q < p1 ? dy[q * dycs + co] : (uint16_t)0;
+    }
+    for (int i = tid; i < 32 * CIN_P; i += 256) {
+      const int pp = i / CIN_P, ci = i % CIN_P;
+      const long q = p + pp;
+      uint16_t v = 0;
+      if (q < p1 && ci < cin) {
+        const int xx = (int)(q % Wd);
+        long r = q / Wd;
+        const int yv = (int)(r % H);
+        r /= H;
+        const int zv = (int)(r % D);
+        const int z2 = zv + dz, y2 = yv + dyy, x2 = xx + dx;
+        if (z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd)
+          v = x[(q + ((long)dz * H + dyy) * Wd + dx) * xcs + ci];
+      }
+      Bs[ci][pp] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) {
+      const int t = wave + 4 * k;
+      if (t < T) {
+        const int mt = t / NTC, nt = t % NTC;
+        const bf16x8 a = ld8(&As[mt * 16 + col][8 * kq]);
+        const bf16x8 b = ld8(&Bs[nt * 16 + col][8 * kq]);
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    const int t = wave + 4 * k;
+    if (t < T) {
+      const int mt = t / NTC, nt = t % NTC;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co = mt * 16 + 4 * kq + j, ci = nt * 16 + col;
+        if (ci < cin) atomicAdd(dW + ((long)tap * COUT + co) * CIN_P + ci, acc[k][j]);
+      }
+    }
+  }
+}
+
+__global__ void k_bias_grad(const uint16_t *__restrict__ dy, int dycs, int c, long npos,
+                            float *db) {
+  // thread t accumulates channel t % c over positions t / c, stride blockDim/c
+  __shared__ float red[256];
+  const int per = 256 / c;  // positions per block pass (c divides 256)
+  const int ch = threadIdx.x % c, sub = threadIdx.x / c;
+  float s = 0.f;
+  if (sub < per)
+    for (long p = (long)blockIdx.x * per + sub; p < npos; p += (long)gridDim.x * per)
+      s += bf2f(dy[p * dycs + ch]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < c) {
+    float t = 0.f;
+    for (int k = 0; k < per; ++k) t += red[k * c + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
+}
+
+__global__ void k_flip(const uint16_t *W, uint16_t *Wt, int cin, int cout) {
+  // W [27][cout][cin_p] -> Wt [27][cin][cout_p]
+  const int cin_p = cpad(cin), cout_p = cpad(cout);
+  const long n = 27L * cin * cout_p;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int co = (int)(i % cout_p);
+    const int ci = (int)((i / cout_p) % cin);
+    const int tap = (int)(i / ((long)cout_p * cin));
+    Wt[i] = co < cout ? W[((long)(26 - tap) * cout + co) * cin_p + ci] : (uint16_t)0;
+  }
+}
+
+__global__ void k_pack(const float *Wf, uint16_t *W, int cin, int cout) {
+  const int cin_p = cpad(cin);
+  const long n = 27L * cout * cin_p;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % cin_p);
+    const long r = i / cin_p;  // tap*cout + co
+    W[i] = ci < cin ? f2bf(Wf[r * cin + ci]) : (uint16_t)0;
+  }
+}
+
+__global__ void k_unpack(const float *gp, float *g, int cin, int cout) {
+  const int cin_p = cpad(cin);
+  const long n = 27L * cout * cin;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % cin);
+    const long r = i / cin;
+    g[i] = gp[r * cin_p + ci];
+  }
+}
+
+// ------------------------------------------------------------- first layer
+__global__ void k_first_fwd(const float *__restrict__ x, const float *__restrict__ W,
+                            const float *__restrict__ b, uint16_t *y, int ycs, int cout, int N,
+                            int D, int H, int Wd) {
+  const long npos = (long)N * D * H * Wd;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (long)gridDim.x * blockDim.x) {
+    const int xx = (int)(p % Wd);
+    long r = p / Wd;
+    const int yv = (int)(r % H);
+    r /= H;
+    const int zv = (int)(r % D);
+    float nb[27];
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+      const int z2 = zv + tap / 9 - 1, y2 = yv + (tap / 3) % 3 - 1, x2 = xx + tap % 3 - 1;
+      const bool ok = z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd;
+      nb[tap] = ok ? x[p + ((long)(tap / 9 - 1) * H + (tap / 3) % 3 - 1) * Wd + tap % 3 - 1] : 0.f;
+    }
+    for (int co = 0; co < cout; ++co) {
+      float s = b[co];
+#pragma unroll
+      for (int tap = 0; tap < 27; ++tap) s = fmaf(W[tap * cout + co], nb[tap], s);
+      y[p * ycs + co] = f2bf(fmaxf(s, 0.f));
+    }
+  }
+}
+
+__global__ void k_first_wgrad(const float *__restrict__ x, const uint16_t *__restrict__ dy,
+                              int dycs, int cout, float *dW, float *db, int N, int D, int H,
+                              int Wd) {
+  // blockIdx.y = tap (27) or 27 for the bias; thread accumulates all channels
+  const int tap = blockIdx.y;
+  const long npos = (long)N * D * H * Wd;
+  float acc[64];
+  for (int c = 0; c < cout; ++c) acc[c] = 0.f;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (long)gridDim.x * blockDim.x) {
+    float xv = 1.f;
+    if (tap < 27) {
+      const int xx = (int)(p % Wd);
+      long r = p / Wd;
+      const int yv = (int)(r % H);
+      r /= H;
+      const int zv = (int)(r % D);
+      const int z2 = zv + tap / 9 - 1, y2 = yv + (tap / 3) % 3 - 1, x2 = xx + tap % 3 - 1;
+      const bool ok = z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd;
+      xv = ok ? x[p + ((long)(tap / 9 - 1) * H + (tap / 3) % 3 - 1) * Wd + tap % 3 - 1] : 0.f;
+    }
+    for (int c = 0; c < cout; ++c) acc[c] = fmaf(bf2f(dy[p * dycs + c]), xv, acc[c]);
+  }
+  __shared__ float red[256];
+  for (int c = 0; c < cout; ++c) {
+    red[threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(tap < 27 ? dW + tap * cout + c : db + c, red[0]);
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------------- head
+__global__ void k_head_fwd(const uint16_t *__restrict__ y, int ycs, int c,
+                           const float *__restrict__ W, const float *__restrict__ b,
+                           const float *__restrict__ x0, float *out, long npos) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (long)gridDim.x * blockDim.x) {
+    float s = b[0] + x0[p];
+    for (int k = 0; k < c; ++k) s = fmaf(W[k], bf2f(y[p * ycs + k]), s);
+    out[p] = s;
+  }
+}
+
+__global__ void k_head_bwd(const uint16_t *__restrict__ y, int ycs, int c,
+                           const float *__restrict__ W, const float *__restrict__ g, float *dy,
+                           int dycs, float *dW, float *db, long npos) {
+  float acc[17];
+  for (int k = 0; k <= c; ++k) acc[k] = 0.f;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (long)gridDim.x * blockDim.x) {
+    const float gv = g[p];
+    for (int k = 0; k < c; ++k) {
+      dy[p * dycs + k] = gv * W[k];
+      acc[k] = fmaf(gv, bf2f(y[p * ycs + k]), acc[k]);
+    }
+    acc[c] += gv;
+  }
+  __shared__ float red[256];
+  for (int k = 0; k <= c; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(k < c ? dW + k : db, red[0]);
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------- elementwise
+__global__ void k_relu_bwd(const float *__restrict__ dy, int dycs,
+                           const uint16_t *__restrict__ y, int ycs, uint16_t *dpre, int dcs,
+                           int c, long npos) {
+  const long n = npos * c;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long p = i / c;
+    const int k = (int)(i % c);
+    const float g = bf2f(y[p * ycs + k]) > 0.f ? dy[p * dycs + k] : 0.f;
+    dpre[p * dcs + k] = f2bf(g);
+  }
+}
+
+__global__ void k_pool_fwd(const uint16_t *__restrict__ x, int xcs, uint16_t *y, int ycs, int c,
+                           int N, int D, int H, int Wd) {
+  const int d2 = D / 2, h2 = H / 2, w2 = Wd / 2;
+  const long n = (long)N * d2 * h2 * w2 * c;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % c);
+    long r = i / c;
+    const int xx = (int)(r % w2);
+    r /= w2;
+    const int yv = (int)(r % h2);
+    r /= h2;
+    const int zv = (int)(r % d2);
+    const int nn = (int)(r / d2);
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      const long q = (((long)nn * D + 2 * zv + (o >> 2)) * H + 2 * yv + ((o >> 1) & 1)) * Wd +
+                     2 * xx + (o & 1);
+      s += bf2f(x[q * xcs + k]);
+    }
+    y[(i / c) * ycs + k] = f2bf(0.125f * s);
+  }
+}
+
+__global__ void k_pool_bwd(const float *__restrict__ dy, int dycs, float *dx, int dxcs, int c,
+                           int N, int D, int H, int Wd) {
+  const long n = (long)N * D * H * Wd * c;  // over fine positions
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % c);
+    long r = i / c;
+    const int xx = (int)(r % Wd);
+    long t = r / Wd;
+    const int yv = (int)(t % H);
+    t /= H;
+    const int zv = (int)(t % D);
+    const int nn = (int)(t / D);
+    const long q = (((long)nn * (D / 2) + zv / 2) * (H / 2) + yv / 2) * (Wd / 2) + xx / 2;
+    dx[r * dxcs + k] += 0.125f * dy[q * dycs + k];
+  }
+}
+
+__global__ void k_up_fwd(const uint16_t *__restrict__ x, int xcs, uint16_t *y, int ycs, int c,
+                         int N, int D, int H, int Wd) {
+  // v = coarse shape; fine = 2x
+  const long n = (long)N * 8 * D * H * Wd * c;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % c);
+    long r = i / c;
+    const int xx = (int)(r % (2 * Wd));
+    long t = r / (2 * Wd);
+    const int yv = (int)(t % (2 * H));
+    t /= 2 * H;
+    const int zv = (int)(t % (2 * D));
+    const int nn = (int)(t / (2 * D));
+    const long q = (((long)nn * D + zv / 2) * H + yv / 2) * Wd + xx / 2;
+    y[r * ycs + k] = x[q * xcs + k];
+  }
+}
+
+__global__ void k_up_bwd(const float *__restrict__ dy, int dycs, float *dx, int dxcs, int c,
+                         int N, int D, int H, int Wd) {
+  const long n = (long)N * D * H * Wd * c;  // over coarse positions
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % c);
+    long r = i / c;
+    const int xx = (int)(r % Wd);
+    long t = r / Wd;
+    const int yv = (int)(t % H);
+    t /= H;
+    const int zv = (int)(t % D);
+    const int nn = (int)(t / D);
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      const long q = (((long)nn * 2 * D + 2 * zv + (o >> 2)) * 2 * H + 2 * yv + ((o >> 1) & 1)) *
+                         2 * Wd + 2 * xx + (o & 1);
+      s += dy[q * dycs + k];
+    }
+    dx[r * dxcs + k] += s;
+  }
+}
+
+__global__ void k_add_ch(const float *__restrict__ src, int scs, float *dst, int dcs, int c,
+                         long npos) {
+  const long n = npos * c;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long p = i / c;
+    const int k = (int)(i % c);
+    dst[p * dcs + k] += src[p * scs + k];
+  }
+}
+
+__global__ void k_adam(float *p, const float *__restrict__ g, float *m, float *v, long n,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2, float wd) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * ((mi / bc1) / (sqrtf(vi / bc2) + eps) + wd * p[i]);
+  }
+}
+
+template <int CIN_P, int COUT>
+hipError_t fwd_t(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, bool relu,
+                 bool f32, hipStream_t st) {
+  const long tiles = (long)v.n * v.d * v.h * (v.w / 16);
+  const int grid = (int)((tiles + 3) / 4);
+  if (f32)
+    k_conv_fwd<CIN_P, COUT, false, true><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+  else if (relu)
+    k_conv_fwd<CIN_P, COUT, true, false><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+  else
+    k_conv_fwd<CIN_P, COUT, false, false><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+template <int CIN_P, int COUT>
+hipError_t wgrad_t(const Act &x, const Act &dy, float *dW, Vol v, hipStream_t st) {
+  const long npos = (long)v.n * v.spatial();
+  const long chunk = 2048;
+  const dim3 grid((unsigned)((npos + chunk - 1) / chunk), 27);
+  k_conv_wgrad<CIN_P, COUT><<<grid, 256, 0, st>>>((const uint16_t *)x.p, x.cs, x.c,
+                                                  (const uint16_t *)dy.p, dy.cs, dW, v.n, v.d,
+                                                  v.h, v.w, chunk);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// The (cin_pad, cout) pairs of the network's convolutions (16->16, 16->32,
+// 32->32, 32->64, 64->64, 96->32, 48->16), of their transposes (input
+// gradients) and of their weight gradients.
+#define BRTM_CONV_PAIRS(X) \
+  X(32, 16) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(64, 16) X(64, 32) X(32, 96) X(32, 48)
+
+hipError_t conv3d_fwd(const Act &x, const uint16_t *W, const float *bias, Act y, Vol v,
+                      bool relu, bool out_f32, hipStream_t st) {
+  if (v.w % 16 || x.c % 8 || x.cs % 8 || y.c % 16) return hipErrorInvalidValue;
+  const int cp = cpad(x.c);
+#define X(CP, CO) \
+  if (cp == CP && y.c == CO) return fwd_t<CP, CO>(x, W, bias, y, v, relu, out_f32, st);
+  BRTM_CONV_PAIRS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t conv3d_wgrad(const Act &x, const Act &dy, float *dW, float *db, Vol v,
+                        hipStream_t st) {
+  if (x.c % 8 || x.cs % 8 || dy.c % 16 || 256 % dy.c) return hipErrorInvalidValue;
+  const int cp = cpad(x.c);
+  hipError_t e = hipErrorInvalidValue;
+#define X(CP, CO) \
+  if (cp == CP && dy.c == CO) e = wgrad_t<CP, CO>(x, dy, dW, v, st);
+  BRTM_CONV_PAIRS(X)
+#undef X
+  if (e != hipSuccess) return e;
+  const long npos = (long)v.n * v.spatial();
+  k_bias_grad<<<512, 256, 0, st>>>((const uint16_t *)dy.p, dy.cs, dy.c, npos, db);
+  return hipGetLastError();
+}
+
+hipError_t conv3d_flip_weights(const uint16_t *W, uint16_t *Wt, int cin, int cout,
+                               hipStream_t st) {
+  k_flip<<<grid1(27L * cin * cpad(cout)), 256, 0, st>>>(W, Wt, cin, cout);
+  return hipGetLastError();
+}
+
+hipError_t pack_weights(const float *Wf, uint16_t *W, int cin, int cout, hipStream_t st) {
+  k_pack<<<grid1(27L * cout * cpad(cin)), 256, 0, st>>>(Wf, W, cin, cout);
+  return hipGetLastError();
+}
+
+hipError_t unpack_grad(const float *gp, float *g, int cin, int cout, hipStream_t st) {
+  k_unpack<<<grid1(27L * cout * cin), 256, 0, st>>>(gp, g, cin, cout);
+  return hipGetLastError();
+}
+
+hipError_t conv3d_first_fwd(const float *x, const float *W, const float *b, Act y, Vol v,
+                            hipStream_t st) {
+  k_first_fwd<<<grid1((long)v.n * v.spatial()), 256, 0, st>>>(x, W, b, (uint16_t *)y.p, y.cs,
+                                                               y.c, v.n, v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t conv3d_first_wgrad(const float *x, const Act &dy, float *dW, float *db, Vol v,
+                              hipStream_t st) {
+  if (dy.c > 64) return hipErrorInvalidValue;
+  k_first_wgrad<<<dim3(256, 28), 256, 0, st>>>(x, (const uint16_t *)dy.p, dy.cs, dy.c, dW, db,
+                                               v.n, v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t head_fwd(const Act &y, const float *W, const float *b, const float *x0, float *out,
+                    Vol v, hipStream_t st) {
+  const long npos = (long)v.n * v.spatial();
+  k_head_fwd<<<grid1(npos), 256, 0, st>>>((const uint16_t *)y.p, y.cs, y.c, W, b, x0, out, npos);
+  return hipGetLastError();
+}
+
+hipError_t head_bwd(const Act &y, const float *W, const float *g, float *dy, int dy_cs,
+                    float *dW, float *db, Vol v, hipStream_t st) {
+  if (y.c > 16) return hipErrorInvalidValue;
+  const long npos = (long)v.n * v.spatial();
+  k_head_bwd<<<512, 256, 0, st>>>((const uint16_t *)y.p, y.cs, y.c, W, g, dy, dy_cs, dW, db,
+                                  npos);
+  return hipGetLastError();
+}
+
+hipError_t relu_bwd(const float *dy, int dy_cs, const Act &y, Act dpre, Vol v, hipStream_t st) {
+  const long npos = (long)v.n * v.spatial();
+  k_relu_bwd<<<grid1(npos * y.c), 256, 0, st>>>(dy, dy_cs, (const uint16_t *)y.p, y.cs,
+                                                (uint16_t *)dpre.p, dpre.cs, y.c, npos);
+  return hipGetLastError();
+}
+
+hipError_t avgpool_fwd(const Act &x, Act y, Vol v, hipStream_t st) {
+  const long n = (long)v.n * (v.d / 2) * (v.h / 2) * (v.w / 2) * x.c;
+  k_pool_fwd<<<grid1(n), 256, 0, st>>>((const uint16_t *)x.p, x.cs, (uint16_t *)y.p, y.cs, x.c,
+                                       v.n, v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t avgpool_bwd(const float *dy, int dy_cs, float *dx, int dx_cs, int c, Vol v,
+                       hipStream_t st) {
+  k_pool_bwd<<<grid1((long)v.n * v.spatial() * c), 256, 0, st>>>(dy, dy_cs, dx, dx_cs, c, v.n,
+                                                                   v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t upsample_fwd(const Act &x, Act y, Vol v, hipStream_t st) {
+  k_up_fwd<<<grid1((long)v.n * 8 * v.spatial() * x.c), 256, 0, st>>>(
+      (const uint16_t *)x.p, x.cs, (uint16_t *)y.p, y.cs, x.c, v.n, v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t upsample_bwd(const float *dy, int dy_cs, float *dx, int dx_cs, int c, Vol v,
+                        hipStream_t st) {
+  k_up_bwd<<<grid1((long)v.n * v.spatial() * c), 256, 0, st>>>(dy, dy_cs, dx, dx_cs, c, v.n,
+                                                                 v.d, v.h, v.w);
+  return hipGetLastError();
+}
+
+hipError_t add_channels(const float *src, int scs, float *dst, int dcs, int c, long npos,
+                        hipStream_t st) {
+  k_add_ch<<<grid1(npos * c), 256, 0, st>>>(src, scs, dst, dcs, c, npos);
+  return hipGetLastError();
+}
+
+hipError_t adam_step(float *p, const float *g, float *m, float *v, long n, float lr, float b1,
+                     float b2, float eps, float bc1, float bc2, float wd, hipStream_t st) {
+  k_adam<<<grid1(n), 256, 0, st>>>(p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, wd);
+  return hipGetLastError();
+}
+
+}  // namespace brtm

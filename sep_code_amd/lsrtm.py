@@ -46,11 +46,20 @@ def crop(w, vol):
     return np.ascontiguousarray(vol[o3:o3 + m3, o2:o2 + m2, o1:o1 + m1])
 
 
+def model_mute(w):
+    """The reference's mute volume (Born/image_rtm_3d.cpp:65-70): zero over
+    the shallow near field, one below."""
+    m1, m2, m3 = model_box(w)[:3]
+    mute = np.ones((m3, m2, m1), np.float32)
+    mute[:w.mute_top] = 0.0
+    return mute
+
+
 def setup_operator(dev: BornDevice, w, store=None, ckpt=None):
     dev.create_space(w.d, w.bc_a, w.bc_b, w.n)
     dev.set_velocity(w.V)
     dev.set_imaging_scale(w.cimg)
-    op = RtmZeroOp(dev, w, store=store, ckpt=ckpt, model_box=model_box(w))
+    op = RtmZeroOp(dev, w, store=store, ckpt=ckpt, model_box=model_box(w), mute=model_mute(w))
     t0 = time.perf_counter()
     op.prepare()
     dev.sync()

@@ -295,10 +295,13 @@ class SimpleSolver:
         self.model = op.model_vector()
         self.verbose = verbose
         self._precond_cb = None
+        self._precond_user = None
         self._precond = precond
-        if precond is not None:
-            vecs = {}
-
+        if isinstance(precond, tuple) and precond[0] == "native":
+            # a C preconditioner exported by the library (e.g. brtm_unet_precond)
+            self._precond_cb = PRECOND_FN(precond[1])
+            self._precond_user = precond[2]
+        elif precond is not None:
             def cb(user, g_h, z_h):
                 try:
                     return int(precond(g_h, z_h) or 0)
@@ -307,12 +310,11 @@ class SimpleSolver:
                     return 1
 
             self._precond_cb = PRECOND_FN(cb)
-            self._vecs = vecs
 
     def solve(self, niter, rel_tol=0.0) -> SolveResult:
         p = CgParams(int(niter), float(rel_tol),
                      self._precond_cb if self._precond_cb is not None else PRECOND_FN(),
-                     None, int(self.verbose))
+                     self._precond_user, int(self.verbose))
         rh = np.zeros(niter + 1)
         th = np.zeros(niter + 1)
         st = CgStats()

@@ -1,0 +1,144 @@
+"""Python face of the learned preconditioner (include/born_unet.h).
+
+``UNet`` owns a network on one device context: forward passes, MSE loss and
+gradients, Adam steps, and whole-volume application on overlapping patches
+as a flexible-CG preconditioner (``UNet.precond`` plugs into
+``SimpleSolver``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, f32, ptr
+from .operator import DeviceVector, _ensure
+
+_P, _I, _L, _D, _F = C.c_void_p, C.c_int, C.c_long, C.c_double, C.c_float
+
+_SIGS = {
+    "brtm_unet_create": [_P, _I, _I, C.c_ulonglong, C.POINTER(_P)],
+    "brtm_unet_destroy": [_P],
+    "brtm_unet_num_params": [_P],
+    "brtm_unet_get_params": [_P, _P],
+    "brtm_unet_set_params": [_P, _P],
+    "brtm_unet_forward": [_P, _P, _P, _I],
+    "brtm_unet_loss_grad": [_P, _P, _P, _I, C.POINTER(_D), _P, _I],
+    "brtm_unet_train_step": [_P, _P, _P, _I, _F, C.POINTER(_D)],
+    "brtm_unet_extract": [_P, _P, _I, _I, _I, _I, _I, _I, _F, _P],
+    "brtm_unet_apply": [_P, _P, _P, _I, _I, _I, _I],
+    "brtm_unet_set_grid": [_P, _I, _I, _I, _I],
+    "brtm_unet_precond": [_P, _P, _P],
+}
+_registered = False
+
+
+def _reg():
+    global _registered
+    _ensure()
+    if not _registered:
+        _lib.register(_SIGS, {"brtm_unet_num_params": _L})
+        _registered = True
+    return _lib.load()
+
+
+class UNet:
+    def __init__(self, dev, batch: int = 4, patch: int = 64, seed: int = 1):
+        lib = _reg()
+        self.dev = dev
+        self.batch, self.patch = int(batch), int(patch)
+        h = _P()
+        call("brtm_unet_create", dev.h, self.batch, self.patch, C.c_ulonglong(seed), C.byref(h))
+        self.h = h
+        self.nparams = int(lib.brtm_unet_num_params(h))
+        self.P3 = self.patch ** 3
+        self.xbuf = DeviceVector(dev, self.batch * self.P3)
+        self.tbuf = DeviceVector(dev, self.batch * self.P3)
+        self.ybuf = DeviceVector(dev, self.batch * self.P3)
+        dev.adopt(self)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            call("brtm_unet_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters -----------------------------------------------------------
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        call("brtm_unet_get_params", self.h, ptr(out))
+        return out
+
+    def set_params(self, theta):
+        theta = f32(theta)
+        assert theta.size == self.nparams
+        call("brtm_unet_set_params", self.h, ptr(theta))
+
+    def save(self, path):
+        np.save(path, self.get_params())
+
+    def load(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
+
+    # -- batches (host arrays [nb][P][P][P]) --------------------------------------
+    def _stage(self, buf, a):
+        a = f32(a).reshape(-1)
+        nb = a.size // self.P3
+        assert nb * self.P3 == a.size and 1 <= nb <= self.batch
+        full = np.zeros(self.batch * self.P3, np.float32)
+        full[:a.size] = a
+        buf.upload(full)
+        return nb
+
+    def forward(self, x):
+        nb = self._stage(self.xbuf, x)
+        call("brtm_unet_forward", self.h, self.xbuf.data_ptr, self.ybuf.data_ptr, nb)
+        return self.ybuf.numpy()[:nb * self.P3].reshape(nb, self.patch, self.patch, self.patch)
+
+    def loss_grad(self, x, t):
+        nb = self._stage(self.xbuf, x)
+        self._stage(self.tbuf, t)
+        loss = _D()
+        g = np.zeros(self.nparams, np.float32)
+        call("brtm_unet_loss_grad", self.h, self.xbuf.data_ptr, self.tbuf.data_ptr, nb,
+             C.byref(loss), ptr(g), 1)
+        return loss.value, g
+
+    def train_step_device(self, nb, lr):
+        """One Adam step on the batch already in xbuf / tbuf (device)."""
+        loss = _D()
+        call("brtm_unet_train_step", self.h, self.xbuf.data_ptr, self.tbuf.data_ptr, int(nb),
+             float(lr), C.byref(loss))
+        return loss.value
+
+    def extract(self, vol: DeviceVector, dims, origin, scale, which, slot):
+        """Copy a P^3 patch of a device volume (dims = (m1, m2, m3)) into
+        slot `slot` of the input ('x') or target ('t') batch."""
+        buf = self.xbuf if which == "x" else self.tbuf
+        dst = buf.data_ptr + 4 * slot * self.P3
+        call("brtm_unet_extract", self.h, vol.data_ptr, int(dims[0]),
+             int(dims[1]), int(dims[2]), int(origin[0]), int(origin[1]), int(origin[2]),
+             float(scale), dst)
+
+    # -- preconditioner -------------------------------------------------------
+    def apply(self, g: DeviceVector, z: DeviceVector, dims, overlap=8):
+        call("brtm_unet_apply", self.h, g.h, z.h, int(dims[0]), int(dims[1]), int(dims[2]),
+             int(overlap))
+
+    def precond(self, dims, overlap=8):
+        """(native function address, user handle) for SimpleSolver."""
+        call("brtm_unet_set_grid", self.h, int(dims[0]), int(dims[1]), int(dims[2]),
+             int(overlap))
+        lib = _reg()
+        addr = C.cast(lib.brtm_unet_precond, C.c_void_p).value
+        return ("native", addr, self.h.value)
+
+
+def _vec_data(v: DeviceVector):
+    return v.data_ptr

@@ -1,0 +1,121 @@
+"""GPU tests of the learned preconditioner: BF16-MFMA U-Net forward and
+gradients against the FP64 oracle (oracle/unet_oracle.py), training, and use
+inside flexible CG."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def net(dev):
+    from sep_code_amd.unet import UNet
+    dev.create_space((10.0, 10.0, 10.0), 4.0, 0.0, (16, 16, 16))
+    u = UNet(dev, batch=2, patch=64, seed=3)
+    yield u
+    u.close()
+
+
+def test_param_layout_matches_oracle(net):
+    assert net.nparams == U.layout()[1]
+
+
+def _theta(net, head=0.5, seed=0):
+    theta = net.get_params().astype(np.float64)
+    lay, _ = U.layout()
+    r = np.random.default_rng(seed)
+    o, _ = lay["wh"]
+    theta[o:o + 16] = r.standard_normal(16) * head
+    for k, (o, s) in lay.items():  # non-zero biases exercise the bias paths
+        if k.startswith("b") and k != "bh":
+            theta[o:o + int(np.prod(s))] = r.standard_normal(int(np.prod(s))) * 0.05
+    return theta
+
+
+def test_forward_matches_fp64_oracle(net):
+    announce("unet forward vs oracle")
+    theta = _theta(net)
+    net.set_params(theta)
+    r = np.random.default_rng(1)
+    x = r.standard_normal((2, 64, 64, 64)).astype(np.float32)
+    y = net.forward(x)
+    yo, _, _ = U.forward(net.get_params(), x)
+    # compare the network part (out - x); BF16 weights and activations
+    e = rel(y - x, yo - x)
+    announce(f"  forward rel err {e:.3e}")
+    assert e < 3e-2
+
+
+def test_gradients_match_fp64_oracle(net):
+    announce("unet gradients vs oracle")
+    theta = _theta(net, seed=2)
+    net.set_params(theta)
+    r = np.random.default_rng(2)
+    x = r.standard_normal((1, 64, 64, 64)).astype(np.float32)
+    t = r.standard_normal((1, 64, 64, 64)).astype(np.float32)
+    loss, g = net.loss_grad(x, t)
+    lo, go, _ = U.loss_grad(net.get_params(), x, t)
+    assert abs(loss - lo) / lo < 1e-2
+    lay, _ = U.layout()
+    for k, (o, s) in lay.items():
+        n = int(np.prod(s))
+        e = rel(g[o:o + n], go[o:o + n])
+        announce(f"  grad {k}: rel err {e:.3e}")
+        assert e < 8e-2, k
+
+
+def test_training_reduces_loss(net):
+    announce("unet training")
+    from sep_code_amd.unet import UNet
+    u = UNet(net.dev, batch=2, patch=64, seed=11)
+    r = np.random.default_rng(5)
+    x = r.standard_normal((2, 64, 64, 64)).astype(np.float32)
+    # target: a smoothed and rescaled copy (learnable by 3x3x3 convolutions)
+    t = 0.5 * x.copy()
+    for ax in (1, 2, 3):
+        t = (t + np.roll(t, 1, ax) + np.roll(t, -1, ax)) / 3.0
+    u._stage(u.xbuf, x)
+    u._stage(u.tbuf, t)
+    losses = [u.train_step_device(2, 1e-3) for _ in range(60)]
+    announce(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.3 * losses[0]
+    u.close()
+
+
+def test_apply_covers_volume_and_is_scale_equivariant(net):
+    from sep_code_amd.operator import DeviceVector
+    net.set_params(_theta(net, head=0.2))
+    dims = (100, 90, 70)
+    r = np.random.default_rng(7)
+    g = r.standard_normal(dims[::-1]).astype(np.float32)
+    gv = DeviceVector.from_array(net.dev, g)
+    zv = DeviceVector(net.dev, g.size)
+    net.apply(gv, zv, dims, overlap=8)
+    z = zv.numpy().reshape(g.shape)
+    assert np.isfinite(z).all() and np.count_nonzero(z) == z.size
+    gv2 = DeviceVector.from_array(net.dev, 3.0 * g)
+    zv2 = DeviceVector(net.dev, g.size)
+    net.apply(gv2, zv2, dims, overlap=8)
+    np.testing.assert_allclose(zv2.numpy(), 3.0 * z.ravel(), rtol=1e-4, atol=1e-4)
+    # the tile with patch origin (40, 40, 40) equals rms * net(patch / rms)
+    # on its central 48^3 (clipped to the volume), zeros outside the volume
+    rms = np.sqrt((g.astype(np.float64) ** 2).mean())
+    patch = np.zeros((64, 64, 64), np.float32)
+    src = g[40:104, 40:104, 40:104]
+    patch[:src.shape[0], :src.shape[1], :src.shape[2]] = src / rms
+    y = net.forward(patch[None])[0] * rms
+    np.testing.assert_allclose(z[48:70, 48:90, 48:96], y[8:30, 8:50, 8:56], rtol=1e-4, atol=1e-4)
