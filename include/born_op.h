@@ -104,10 +104,16 @@ typedef int (*brtm_precond_fn)(void *user, const brtm_vec *g, brtm_vec *z);
 
 typedef struct {
   int niter;                /* maximum iterations (lin_solver::solve(niter))    */
-  double rel_tol;           /* stop when |Lm-d|/|d| <= rel_tol (0: never)       */
+  double rel_tol;           /* stop when the judged residual <= rel_tol (0: never) */
   brtm_precond_fn precond;  /* NULL: plain cgstep                               */
   void *precond_user;
   int verbose;
+  /* Optional data-space offset e (NULL: none).  The judged residual is then
+   * |rr + e| / |d - e| instead of |rr| / |d|: with e the additive noise of
+   * synthetic data d = L m_true + e, it is the relative misfit to the
+   * noise-free data (|L m - d_clean| / |d_clean|).  The iteration itself
+   * still minimises |L m - d|. */
+  const brtm_vec *offset;
 } brtm_cg_params;
 
 typedef struct {

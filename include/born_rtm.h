@@ -166,6 +166,12 @@ int brtm_kernel_timing_result(brtm_ctx *ctx, double *total_ms, int *count);
  * default) and target workgroup count (used when ty_tile != 0). */
 int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 
+/* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
+ * (16-byte vector loads and stores) between two device buffers; avg_ms
+ * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE
+ * scale the counters of the other kernels to HBM bytes. */
+int brtm_copy_bench(brtm_ctx *ctx, double bytes, int nrep, double *avg_ms);
+
 /* Device synchronisation of the library stream. */
 int brtm_sync(brtm_ctx *ctx);
 

@@ -181,7 +181,7 @@ def rtm512(seed: int = 4096, nt: int = 4000) -> Workload:
 
 
 def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_step=2,
-          d=12.5, jimg=4, store="bf16") -> Workload:
+          d=12.5, jimg=4, store="bf16", anomalies: int = 0) -> Workload:
     n = (n_side, n_side, n_side)
     dd, dt, sponge = (d, d, d), 0.001, max(8, n_side // 12)
     nbound = sponge + 4
@@ -208,6 +208,15 @@ def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_
         amp = sgn * (vels[k + 1] - vels[k]) / (vels[k + 1] + vels[k])
         jj, ii = np.meshgrid(np.arange(n2), np.arange(n1), indexing="ij")
         refl[iz, jj, ii] += np.float32(amp)
+    # Gaussian velocity anomalies (sigma 5-15 % of the grid, +-15 %)
+    for _ in range(anomalies):
+        cx, cy = rng.uniform(nbound, n1 - nbound), rng.uniform(nbound, n2 - nbound)
+        cz = rng.uniform(top, bot)
+        sig = rng.uniform(0.05, 0.15) * n_side
+        amp = rng.uniform(-0.15, 0.15)
+        g = np.exp(-(((x + n1 / 2 - cx) ** 2)[None] + ((y + n2 / 2 - cy) ** 2)[None] +
+                     (zz - cz) ** 2) / (2 * sig * sig))
+        vel *= (1 + amp * g).astype(np.float32)
     u = rng.uniform_array(n1 * n2 * n3).reshape(n3, n2, n1)
     s = rng.uniform_array(n1 * n2 * n3).reshape(n3, n2, n1)
     sparse = (u < 0.002).astype(np.float32) * np.where(s < 0.5, -0.05, 0.05).astype(np.float32)

@@ -767,6 +767,25 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   return BRTM_OK;
 }
 
+int brtm_copy_bench(brtm_ctx *c, double bytes, int nrep, double *avg_ms) {
+  BRTM_REQUIRE(c && avg_ms && nrep >= 1 && bytes >= 16, BRTM_EINVAL, "bad arguments");
+  BRTM_HIP(hipSetDevice(c->device));
+  const long n16 = (long)(bytes / 16);
+  DevBuf a, b;
+  BRTM_TRY(a.alloc(16 * (size_t)n16));
+  BRTM_TRY(b.alloc(16 * (size_t)n16));
+  BRTM_HIP(hipMemsetAsync(a.ptr, 0, 16 * (size_t)n16, c->stream));
+  BRTM_HIP(launch_copy16(a.ptr, b.ptr, n16, c->stream));  // warm-up
+  BRTM_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < nrep; ++r) BRTM_HIP(launch_copy16(a.ptr, b.ptr, n16, c->stream));
+  BRTM_HIP(hipEventRecord(c->ev1, c->stream));
+  BRTM_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  BRTM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *avg_ms = ms / nrep;
+  return BRTM_OK;
+}
+
 int brtm_set_born_path(brtm_ctx *c, int direct_lv) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
   c->born_direct_lv = direct_lv != 0;

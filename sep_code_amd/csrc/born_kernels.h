@@ -104,6 +104,8 @@ hipError_t launch_trace_to_data(const float *tr, float *data, int nrec, int ntb,
                                 float inv_dt, int add, hipStream_t st);
 hipError_t launch_data_to_trace(const float *data, float *tr, int nrec, int ntb, int nt,
                                 float inv_dt, hipStream_t st);
+// Streaming copy of n16 16-byte words (calibration of bandwidth counters).
+hipError_t launch_copy16(const void *src, void *dst, long n16, hipStream_t st);
 // Plain transpose [rows][cols] -> [cols][rows].
 hipError_t launch_transpose(const float *in, float *out, long rows, long cols,
                             hipStream_t st);

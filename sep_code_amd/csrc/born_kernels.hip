@@ -522,6 +522,12 @@ __global__ void k_data_to_trace(const float *__restrict__ data, float *tr, int n
   }
 }
 
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *dst, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_transpose(const float *__restrict__ in, float *out, long rows, long cols) {
   __shared__ float tile[32][33];
   const long c0 = (long)blockIdx.x * 32, r0 = (long)blockIdx.y * 32;
@@ -695,6 +701,11 @@ hipError_t launch_data_to_trace(const float *data, float *tr, int nrec, int ntb,
                                 float inv_dt, hipStream_t st) {
   dim3 grid((nrec + 31) / 32, (ntb + 31) / 32);
   k_data_to_trace<<<grid, dim3(32, 8), 0, st>>>(data, tr, nrec, ntb, nt, inv_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(const void *src, void *dst, long n16, hipStream_t st) {
+  k_copy16<<<8192, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
   return hipGetLastError();
 }
 

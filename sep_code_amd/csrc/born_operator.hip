@@ -2,6 +2,7 @@
 // operator (rtm_zero_op) and the cgstep / simple_solver CG driver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -431,9 +432,39 @@ int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
     return rc;
   }
   const double dnorm = std::sqrt(dd);
-  if (resid_hist) resid_hist[0] = 1.0;
-  if (time_hist) time_hist[0] = 0.0;
+  // Judged residual: |rr + e| / |d - e| with the optional offset e.
+  const brtm_vec *e = prm->offset;
+  double ee = 0, de = 0;
+  if (e) {
+    rc = check_vec(e, nd, "offset");
+    if (rc == BRTM_OK) rc = brtm_vec_dot(e, e, &ee);
+    if (rc == BRTM_OK) rc = brtm_vec_dot(data, e, &de);
+    if (rc != BRTM_OK) {
+      cleanup();
+      return rc;
+    }
+  }
+  const double jnorm = e ? std::sqrt(std::max(dd - 2 * de + ee, 0.0)) : dnorm;
+  auto judged = [&](double rr_rr, double *out) -> int {
+    if (!e) {
+      *out = std::sqrt(rr_rr) / dnorm;
+      return BRTM_OK;
+    }
+    double re = 0;
+    int r2 = brtm_vec_dot(rr, e, &re);
+    *out = std::sqrt(std::max(rr_rr + 2 * re + ee, 0.0)) / jnorm;
+    return r2;
+  };
   S.rel_resid = 1.0;
+  if (e) {
+    rc = judged(dd, &S.rel_resid);  // rr = -d at m = 0
+    if (rc != BRTM_OK) {
+      cleanup();
+      return rc;
+    }
+  }
+  if (resid_hist) resid_hist[0] = S.rel_resid;
+  if (time_hist) time_hist[0] = 0.0;
   bool first = true;  // cgstep::forget (Born/cgstep.cpp:16-26)
   const double small = 1e-20;
   for (int iter = 0; iter < prm->niter && rc == BRTM_OK; ++iter) {
@@ -500,7 +531,8 @@ int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
     if (rc != BRTM_OK) break;
     first = false;
     S.iterations = iter + 1;
-    S.rel_resid = std::sqrt(rr_rr) / dnorm;
+    rc = judged(rr_rr, &S.rel_resid);
+    if (rc != BRTM_OK) break;
     if (resid_hist) resid_hist[iter + 1] = S.rel_resid;
     if (time_hist) time_hist[iter + 1] = elapsed();
     if (prm->verbose)

@@ -156,6 +156,11 @@ class BornDevice:
         call("brtm_time_step", self.h, int(form), int(act), int(fmt), int(nrep), C.byref(ms))
         return ms.value
 
+    def copy_bench(self, nbytes, nrep=10):
+        ms = C.c_double()
+        call("brtm_copy_bench", self.h, float(nbytes), int(nrep), C.byref(ms))
+        return ms.value
+
     def set_born_path(self, direct_lv):
         call("brtm_set_born_path", self.h, int(bool(direct_lv)))
 

@@ -66,31 +66,40 @@ def setup_operator(dev: BornDevice, w, store=None, ckpt=None):
     return op, time.perf_counter() - t0
 
 
-def observed_data(op, w, noise_frac=0.01, seed=1234):
-    """d = L m_true + noise, noise sigma = noise_frac * max |L m_true|."""
+def observed_data(op, w, noise_frac=0.01, seed=1234, keep_noise=False):
+    """d = L m_true + e with Gaussian e of sigma = noise_frac * RMS(L m_true)
+    (|e| / |L m_true| = noise_frac).  Returns (d, floor[, e]) with floor =
+    |e| / |d|, the misfit of the noise-free data itself."""
     mt = op.model_vector(crop(w, w.refl))
     d = op.data_vector()
     op.forward(False, mt, d)
     clean = d.numpy()
     rng = configs.SplitMix64(seed)
-    sigma = noise_frac * float(np.abs(clean).max())
+    sigma = noise_frac * float(np.sqrt((clean.astype(np.float64) ** 2).mean()))
     noise = (rng.normal_array(clean.size) * sigma).astype(np.float32)
     d.upload(clean + noise)
     floor = float(np.linalg.norm(noise.astype(np.float64)) /
                   np.linalg.norm((clean + noise).astype(np.float64)))
+    if keep_noise:
+        return d, floor, op.data_vector(noise)
     return d, floor
 
 
 def run_lsrtm(dev, w, op, d, niter=60, rel_tol=1e-2, precond=None, name="cg", verbose=False,
-              floor=0.0, prepare_seconds=0.0):
+              floor=0.0, prepare_seconds=0.0, noise=None):
+    """Solve and judge.  With noise e given, the residual judged (for the
+    stop and the final check) is the misfit to the noise-free data d - e."""
     solver = SimpleSolver(cgstep(), d, op, precond=precond, verbose=verbose)
-    res = solver.solve(niter, rel_tol=rel_tol)
+    res = solver.solve(niter, rel_tol=rel_tol, offset=noise)
     # judge: fresh plain-operator residual with FP64 norms
     m = solver.return_model()
     r = op.data_vector()
     op.forward(False, m, r)
-    r.scale_add(1.0, d, -1.0)
-    rel = float(np.sqrt(r.dot(r) / d.dot(d)))
+    ref = d
+    if noise is not None:
+        ref = d.clone().scale_add(1.0, noise, -1.0)
+    r.scale_add(1.0, ref, -1.0)
+    rel = float(np.sqrt(r.dot(r) / ref.dot(ref)))
     return LsrtmResult(name, res.iterations, res.converged, float(res.time_hist[-1]),
                        res.rel_resid, rel, [float(x) for x in res.resid_hist],
                        [float(x) for x in res.time_hist], res.fallbacks, prepare_seconds,

@@ -39,7 +39,7 @@ PRECOND_FN = C.CFUNCTYPE(_I, _P, _P, _P)
 
 class CgParams(C.Structure):
     _fields_ = [("niter", _I), ("rel_tol", _D), ("precond", PRECOND_FN), ("precond_user", _P),
-                ("verbose", _I)]
+                ("verbose", _I), ("offset", _P)]
 
 
 class CgStats(C.Structure):
@@ -311,10 +311,13 @@ class SimpleSolver:
 
             self._precond_cb = PRECOND_FN(cb)
 
-    def solve(self, niter, rel_tol=0.0) -> SolveResult:
+    def solve(self, niter, rel_tol=0.0, offset: DeviceVector | None = None) -> SolveResult:
+        """offset: data-space vector e; the judged residual is |rr + e|/|d - e|
+        (with e the synthetic noise: the misfit to the noise-free data)."""
         p = CgParams(int(niter), float(rel_tol),
                      self._precond_cb if self._precond_cb is not None else PRECOND_FN(),
-                     self._precond_user, int(self.verbose))
+                     self._precond_user, int(self.verbose),
+                     offset.h if offset is not None else None)
         rh = np.zeros(niter + 1)
         th = np.zeros(niter + 1)
         st = CgStats()

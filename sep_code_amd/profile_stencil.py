@@ -1,0 +1,53 @@
+"""Kernel-level timing driver for profiling (rocprofv3) at a fixed size.
+
+    python -m sep_code_amd.profile_stencil --n 512 --reps 10
+
+runs, on one context: a plain 16-byte streaming copy of --copy-gib GiB (the
+calibration kernel whose FETCH_SIZE / WRITE_SIZE per byte scale the other
+kernels' counters), then --reps launches each of the stencil variants
+  VL plain step, VL + fused imaging (FP32 snapshot), VL + fused Born source
+  (BF16 snapshot), VL + fused imaging (BF16 snapshot)
+and prints their HIP-event mean durations and algorithmic bandwidths.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+import numpy as np
+
+from .device import BornDevice
+
+# (name, form, act, fmt, algorithmic bytes per cell)
+VARIANTS = [("vl", 0, 0, 0, 16), ("vl_img_f32", 0, 1, 1, 28), ("vl_born_bf16", 0, 2, 2, 22),
+            ("vl_img_bf16", 0, 1, 2, 26)]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--copy-gib", type=float, default=1.0)
+    ap.add_argument("--ty", type=int, default=0)
+    ap.add_argument("--variants", default="all")
+    a = ap.parse_args()
+    n = a.n
+    dev = BornDevice(0)
+    dev.set_tuning(a.ty, 2048)
+    dev.create_space((12.5, 12.5, 12.5), 32.0, 0.001, (n, n, n))
+    dev.set_velocity(np.full((n, n, n), (3000 * 0.001) ** 2, np.float32))
+    out = {"n": n, "reps": a.reps}
+    nbytes = a.copy_gib * 2 ** 30
+    ms = dev.copy_bench(nbytes, a.reps)
+    out["copy"] = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / ms / 1e6}
+    for name, form, act, fmt, B in VARIANTS:
+        if a.variants != "all" and name not in a.variants.split(","):
+            continue
+        ms = dev.time_step(form, act, fmt, a.reps)
+        out[name] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6, "gpts": n ** 3 / ms / 1e6}
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

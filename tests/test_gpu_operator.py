@@ -179,6 +179,32 @@ def test_cg_model_matches_numpy_cgstep(dev):
     op.close()
 
 
+def test_cg_offset_judges_noise_free_misfit(dev):
+    """With offset e the solver reports |L m - (d - e)| / |d - e| (checked
+    against a fresh application of the operator)."""
+    w = small_workload(nt=60, n_side=32)
+    mbox, mu = setup(dev, w)
+    op = RtmZeroOp(dev, w, store="f32", model_box=mbox, mute=mu)
+    op.prepare()
+    r = np.random.default_rng(4)
+    mv = op.model_vector(r.standard_normal(op.model_size))
+    dv = op.data_vector()
+    op.forward(False, mv, dv)
+    clean = dv.numpy().astype(np.float64)
+    e = (0.05 * np.sqrt((clean ** 2).mean()) * r.standard_normal(clean.size)).astype(np.float32)
+    dv.upload(clean + e)
+    ev = op.data_vector(e)
+    solver = SimpleSolver(cgstep(), dv, op)
+    res = solver.solve(4, offset=ev)
+    lm = op.data_vector()
+    op.forward(False, solver.return_model(), lm)
+    ref = np.linalg.norm(lm.numpy().astype(np.float64) - clean) / np.linalg.norm(clean)
+    assert abs(res.rel_resid - ref) < 1e-4 * max(ref, 1e-3) + 1e-6
+    assert res.resid_hist[0] == pytest.approx(np.linalg.norm(clean + e - e) /
+                                              np.linalg.norm(clean), rel=1e-5)
+    op.close()
+
+
 def test_cg_zero_data_breaks_down_cleanly(dev):
     w = small_workload(nt=40, n_side=32)
     mbox, mu = setup(dev, w)
