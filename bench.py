@@ -40,10 +40,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup_device(w, ty, target):
+def setup_device(w, ty, target, xl):
     from sep_code_amd.device import BornDevice
     dev = BornDevice(0)
     dev.set_tuning(ty, target)
+    dev.set_x_per_lane(xl)
     shot = w.shots[0]
     sinc = np.zeros((1, 8), np.float32)
     sinc[0, 3] = 1
@@ -106,6 +107,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="override nt (testing only)")
     ap.add_argument("--ty", type=int, default=0, help="tile rows (0 = per-variant default)")
     ap.add_argument("--target-blocks", type=int, default=2048)
+    ap.add_argument("--x-per-lane", type=int, default=1, choices=(1, 2),
+                    help="cells per lane of the forward-form step kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
@@ -123,7 +126,7 @@ def main():
     kw = {"nt": a.nt} if a.nt else {}
     w = configs.WORKLOADS[a.workload](**kw)
     log(f"workload {w.name} n={w.n} nt={w.nt} generated in {time.perf_counter() - t0:.1f}s")
-    dev = setup_device(w, a.ty, a.target_blocks)
+    dev = setup_device(w, a.ty, a.target_blocks, a.x_per_lane)
     log(f"setup done ({dev.device_bytes() / 2**30:.1f} GiB device) "
         f"{time.perf_counter() - t0:.1f}s")
 
@@ -154,7 +157,8 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (a.traffic_bytes or None),
-            "kernel": "k_step<VL,fused sponge> (brtm::k_step FORM_VL FMT_NONE)",
+            "kernel": ("brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)" if a.x_per_lane == 2
+                       else "brtm::k_step (VL, fused sponge)"),
             "kernel_ms": round(k_ms, 4), "launches": kcount,
             "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
     cpu = None

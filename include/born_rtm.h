@@ -166,6 +166,11 @@ int brtm_kernel_timing_result(brtm_ctx *ctx, double *total_ms, int *count);
  * default) and target workgroup count (used when ty_tile != 0). */
 int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 
+/* Cells per lane along x for VL-form steps: 1 (64-wide tiles, 4-byte
+ * accesses) or 2 (128-wide tiles, 8-byte accesses; used only when n1 is
+ * even, other grids keep the 1-cell kernel).  Results are identical. */
+int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
+
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms
  * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE

@@ -88,12 +88,14 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   // Tile rows / workgroup target per variant (measured on MI355X at 512^3):
   // plain steps prefer tall tiles, 16-bit snapshot variants short ones.
   int ty = c->ty_tile, tb = c->target_blocks;
+  const bool x2 = c->vec2 && form == FORM_VL && (c->n1 % 2) == 0;
   if (ty == 0) {
-    if (act == ACT_NONE) { ty = 32; tb = 1024; }
+    if (x2) { ty = 16; tb = 2048; }
+    else if (act == ACT_NONE) { ty = 32; tb = 1024; }
     else if (fmt == FMT_F32) { ty = 16; tb = 8192; }
     else { ty = 8; tb = 8192; }
   }
-  plan_step(P, ty, tb);
+  plan_step(P, ty, tb, x2 ? 128 : 64);
   const bool timed =
       c->ktime_on && form == c->ktime_form && act == c->ktime_act && fmt == c->ktime_fmt;
   if (timed) {
@@ -106,7 +108,10 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream));
+  if (x2)
+    BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
+  else
+    BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream));
   if (timed) {
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
     c->ktime_used += 2;
@@ -764,6 +769,13 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   BRTM_REQUIRE(target_blocks >= 1, BRTM_EINVAL, "target_blocks must be >= 1");
   c->ty_tile = ty_tile;
   c->target_blocks = target_blocks;
+  return BRTM_OK;
+}
+
+int brtm_set_x_per_lane(brtm_ctx *c, int cells) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(cells == 1 || cells == 2, BRTM_EINVAL, "cells per lane must be 1 or 2");
+  c->vec2 = cells == 2;
   return BRTM_OK;
 }
 

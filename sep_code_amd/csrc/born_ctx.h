@@ -71,6 +71,7 @@ struct brtm_ctx {
   std::vector<float> wtab;
   float cimg = 1e-6f;
   int ty_tile = 0, target_blocks = 2048;  // ty_tile 0: chosen per step variant
+  bool vec2 = false;                       // VL steps with two x per lane (n1 even)
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)

@@ -39,8 +39,11 @@ struct StepParams {
 };
 
 hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
-// Tile / chunk geometry chosen for a grid (fills ntx..zc of P).
-void plan_step(StepParams &P, int ty_tile, int target_blocks);
+// The VL step with two consecutive x per lane (128-wide tiles, n1 even).
+hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// Tile / chunk geometry chosen for a grid (fills ntx..zc of P); xw is the
+// tile width in x (64 for launch_step, 128 for launch_step_x2).
+void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw = 64);
 
 // p[loc_j] += scale * sum_k sinc8[k] tr(j, id+k), tr(j,t) = tr[j*sp + t*st].
 // When img != null: img[loc_j] += cimg * s[loc_j] * (added amount).

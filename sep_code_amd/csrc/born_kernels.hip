@@ -264,6 +264,219 @@ k_step(const StepParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// The VL step with two consecutive x per lane (n1 even): a wave row covers
+// 128 floats with 8-byte loads and stores, and the x taps of the two cells
+// come from four 8-byte LDS reads instead of sixteen 4-byte ones.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__device__ __forceinline__ float2 snap2_at(const void *s, const float *sscale, int i, int z) {
+  if constexpr (FMT == FMT_F32) {
+    return *reinterpret_cast<const float2 *>(static_cast<const float *>(s) + i);
+  } else if constexpr (FMT == FMT_BF16) {
+    const uint32_t u = *reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(s) + i);
+    const float sc = sscale[z];
+    return make_float2(bf16_to_f32((uint16_t)(u & 0xffffu)) * sc, bf16_to_f32((uint16_t)(u >> 16)) * sc);
+  } else if constexpr (FMT == FMT_F16) {
+    const __half2 h = *reinterpret_cast<const __half2 *>(static_cast<const __half *>(s) + i);
+    const float2 f = __half22float2(h);
+    const float sc = sscale[z];
+    return make_float2(f.x * sc, f.y * sc);
+  } else {
+    return make_float2(0.f, 0.f);
+  }
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stream_store2(float *dst, float a, float b) {
+  f32x2 v;
+  v.x = a;
+  v.y = b;
+  __builtin_nontemporal_store(v, reinterpret_cast<f32x2 *>(dst));
+}
+
+template <int TY, int RPT, int FMT, int ACT>
+__global__ void __launch_bounds__(64 * (TY / RPT))
+k_step_x2(const StepParams P) {
+  static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  constexpr int NWV = TY / RPT;
+  constexpr int HY = 8 / NWV;
+  static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
+  constexpr int LW = 136;  // 4 + 128 + 4
+  __shared__ __attribute__((aligned(16))) float plane[2][TY + 8][LW];
+  __shared__ float wsh[kWMax];
+
+  const int lane = threadIdx.x;
+  const int wy = threadIdx.y;
+  const int n1 = P.n1, n2 = P.n2, n3 = P.n3, nw = P.nw;
+  const int s3 = n1 * n2;
+  for (int e = wy * 64 + lane; e < nw; e += 64 * NWV) wsh[e] = P.wtab[e];
+
+  const int nb = P.ntx * P.nty * P.ntz;
+  const int bid = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = bid & 7, jj = bid >> 3;
+  const int t = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const int tx = t % P.ntx;
+  const int ty = (t / P.ntx) % P.nty;
+  const int tz = t / (P.ntx * P.nty);
+
+  const int x = tx * 128 + 2 * lane;  // cells x and x + 1 (x even, n1 even)
+  const int y0 = ty * TY;
+  const int yb = y0 + wy * RPT;
+  const int z0 = 4 + tz * P.zc;
+  const int z1 = min(z0 + P.zc, n3 - 4);
+  const bool xmem = x < n1;
+  const bool xin = (x >= 4) && (x < n1 - 4);  // same for x + 1 since both are even-aligned
+  const int ex0 = edist(x, n1), ex1 = edist(x + 1, n1);
+
+  auto ld2 = [&](const float *a, int i, bool ok) -> float2 {
+    return ok ? *reinterpret_cast<const float2 *>(a + i) : make_float2(0.f, 0.f);
+  };
+
+  float2 q[RPT][9];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = yb + r;
+    const bool ok = xmem && y < n2;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q[r][k] = ld2(P.p, (z0 - 4 + k) * s3 + y * n1 + x, ok);
+  }
+
+  int hyy[HY], hyrow[HY];
+#pragma unroll
+  for (int j = 0; j < HY; ++j) {
+    const int h = wy * HY + j;
+    hyy[j] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+    hyrow[j] = h < 4 ? h : TY + h;
+  }
+  // x halo: lanes 0,1 load columns -4..-1, lanes 2,3 load 128..131 (8 bytes each)
+  const bool hxl = lane < 4;
+  const int hxx = lane < 2 ? tx * 128 - 4 + 2 * lane : tx * 128 + 128 + 2 * (lane - 2);
+  const int hxcol = lane < 2 ? 2 * lane : 132 + 2 * (lane - 2);
+  const bool hxok = hxl && hxx >= 0 && hxx < n1;
+
+  float2 hy[HY], hx[RPT];
+  float2 cpo[RPT], cV[RPT], cS[RPT], cX[RPT];
+
+  auto load_halo = [&](int z, float2 *ohy, float2 *ohx) {
+#pragma unroll
+    for (int j = 0; j < HY; ++j)
+      ohy[j] = ld2(P.p, z * s3 + hyy[j] * n1 + x, xmem && hyy[j] >= 0 && hyy[j] < n2);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      ohx[r] = ld2(P.p, z * s3 + (yb + r) * n1 + hxx, hxok && (yb + r) < n2);
+  };
+  auto load_center = [&](int z, float2 *opo, float2 *oV, float2 *oS, float2 *oX) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const bool ok = xmem && y < n2;
+      const int i = z * s3 + y * n1 + x;
+      opo[r] = ld2(P.po, i, ok);
+      oV[r] = ld2(P.V, i, ok);
+      if constexpr (ACT != ACT_NONE) {
+        oS[r] = ok ? snap2_at<FMT>(P.snap, P.sscale, i, z) : make_float2(0.f, 0.f);
+        oX[r] = ld2(ACT == ACT_IMG ? P.img : P.m, i, ok);
+      }
+    }
+  };
+
+  load_halo(z0, hy, hx);
+  load_center(z0, cpo, cV, cS, cX);
+
+  const float c0 = P.c[0];
+  for (int z = z0; z < z1; ++z) {
+    float(*pl)[LW] = plane[z & 1];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      *reinterpret_cast<float2 *>(&pl[4 + wy * RPT + r][4 + 2 * lane]) = q[r][4];
+#pragma unroll
+    for (int j = 0; j < HY; ++j) *reinterpret_cast<float2 *>(&pl[hyrow[j]][4 + 2 * lane]) = hy[j];
+    if (hxl) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+        *reinterpret_cast<float2 *>(&pl[4 + wy * RPT + r][hxcol]) = hx[r];
+    }
+
+    const int zn = z + 1;
+    float2 nq[RPT], nhy[HY], nhx[RPT];
+    float2 npo[RPT], nV[RPT], nS[RPT], nX[RPT];
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int y = yb + r;
+        nq[r] = ld2(P.p, (zn + 4) * s3 + y * n1 + x, xmem && y < n2);
+      }
+      load_halo(zn, nhy, nhx);
+      load_center(zn, npo, nV, nS, nX);
+    }
+    __syncthreads();
+
+    const int ez = edist(z, n3);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const int row = 4 + wy * RPT + r;
+      const int col = 4 + 2 * lane;
+      const float2 cc = q[r][4];
+      const float2 L0 = *reinterpret_cast<const float2 *>(&pl[row][col - 4]);  // x-4, x-3
+      const float2 L1 = *reinterpret_cast<const float2 *>(&pl[row][col - 2]);  // x-2, x-1
+      const float2 R0 = *reinterpret_cast<const float2 *>(&pl[row][col + 2]);  // x+2, x+3
+      const float2 R1 = *reinterpret_cast<const float2 *>(&pl[row][col + 4]);  // x+4, x+5
+      // axis-1 taps k = 1..4 for cell a (x) and cell b (x + 1)
+      const float xa[4] = {cc.y + L1.y, R0.x + L1.x, R0.y + L0.y, R1.x + L0.x};
+      const float xb[4] = {R0.x + cc.x, R0.y + L1.y, R1.x + L1.x, R1.y + L0.y};
+      float acc_a = c0 * cc.x, acc_b = c0 * cc.y;
+#pragma unroll
+      for (int k = 1; k <= 4; ++k) {
+        const float2 up = *reinterpret_cast<const float2 *>(&pl[row - k][col]);
+        const float2 dn = *reinterpret_cast<const float2 *>(&pl[row + k][col]);
+        const float c1 = P.c[1 + 3 * (k - 1)], c2 = P.c[2 + 3 * (k - 1)], c3 = P.c[3 + 3 * (k - 1)];
+        acc_a = fmaf(c1, xa[k - 1], acc_a);
+        acc_b = fmaf(c1, xb[k - 1], acc_b);
+        acc_a = fmaf(c2, up.x + dn.x, acc_a);
+        acc_b = fmaf(c2, up.y + dn.y, acc_b);
+        acc_a = fmaf(c3, q[r][4 - k].x + q[r][4 + k].x, acc_a);
+        acc_b = fmaf(c3, q[r][4 - k].y + q[r][4 + k].y, acc_b);
+      }
+      if (xin && y >= 4 && y < n2 - 4) {
+        const int eyz = min(edist(y, n2), ez);
+        const int e0 = min(ex0, eyz), e1 = min(ex1, eyz);
+        const float w0 = e0 < nw ? wsh[e0] : 1.f, w1 = e1 < nw ? wsh[e1] : 1.f;
+        const int i = z * s3 + y * n1 + x;
+        float2 pn;
+        pn.x = w0 * fmaf(cV[r].x, acc_a, 2.f * cc.x) - (w0 * w0) * cpo[r].x;
+        pn.y = w1 * fmaf(cV[r].y, acc_b, 2.f * cc.y) - (w1 * w1) * cpo[r].y;
+        if constexpr (ACT == ACT_BORN) {
+          pn.x = fmaf(P.cimg * cS[r].x, cX[r].x, pn.x);
+          pn.y = fmaf(P.cimg * cS[r].y, cX[r].y, pn.y);
+        }
+        stream_store2(P.pn + i, pn.x, pn.y);
+        if constexpr (ACT == ACT_IMG) {
+          stream_store2(P.img + i, fmaf(P.cimg * cS[r].x, pn.x, cX[r].x),
+                        fmaf(P.cimg * cS[r].y, pn.y, cX[r].y));
+        }
+      }
+    }
+
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[r][k] = q[r][k + 1];
+        q[r][8] = nq[r];
+        cpo[r] = npo[r];
+        cV[r] = nV[r];
+        cS[r] = nS[r];
+        cX[r] = nX[r];
+        hx[r] = nhx[r];
+      }
+#pragma unroll
+      for (int j = 0; j < HY; ++j) hy[j] = nhy[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Injection / extraction (one thread per point).
 // ---------------------------------------------------------------------------
 __global__ void k_inject(const InjectParams P) {
@@ -549,6 +762,32 @@ int grid_for(long n, int bs) {
   return (int)g;
 }
 
+template <int TY, int RPT, int ACT>
+hipError_t launch_x2_a(int fmt, const StepParams &P, hipStream_t st) {
+  const dim3 block(64, TY / RPT);
+  const dim3 grid(P.ntx * P.nty * P.ntz);
+  switch (fmt) {
+    case FMT_F32: k_step_x2<TY, RPT, FMT_F32, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_BF16: k_step_x2<TY, RPT, FMT_BF16, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_F16: k_step_x2<TY, RPT, FMT_F16, ACT><<<grid, block, 0, st>>>(P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int TY, int RPT>
+hipError_t launch_x2_t(int act, int fmt, const StepParams &P, hipStream_t st) {
+  if (act == ACT_NONE) {
+    if (fmt != FMT_NONE) return hipErrorInvalidValue;
+    k_step_x2<TY, RPT, FMT_NONE, ACT_NONE><<<dim3(P.ntx * P.nty * P.ntz), dim3(64, TY / RPT), 0,
+                                             st>>>(P);
+    return hipGetLastError();
+  }
+  if (act == ACT_BORN) return launch_x2_a<TY, RPT, ACT_BORN>(fmt, P, st);
+  if (act == ACT_IMG) return launch_x2_a<TY, RPT, ACT_IMG>(fmt, P, st);
+  return hipErrorInvalidValue;
+}
+
 template <int FORM, int TY, int RPT, int ACT>
 hipError_t launch_step_a(int fmt, const StepParams &P, hipStream_t st) {
   const dim3 block(64, TY / RPT);
@@ -579,8 +818,15 @@ hipError_t launch_step_t(int act, int fmt, const StepParams &P, hipStream_t st) 
 
 }  // namespace
 
-void plan_step(StepParams &P, int ty_tile, int target_blocks) {
-  P.ntx = (P.n1 + 63) / 64;
+hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+  if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 10 || (P.n1 & 1)) return hipErrorInvalidValue;
+  if (ty_tile == 32) return launch_x2_t<32, 4>(act, fmt, P, st);
+  if (ty_tile == 8) return launch_x2_t<8, 2>(act, fmt, P, st);
+  return launch_x2_t<16, 2>(act, fmt, P, st);
+}
+
+void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw) {
+  P.ntx = (P.n1 + xw - 1) / xw;
   P.nty = (P.n2 + ty_tile - 1) / ty_tile;
   const int nz = P.n3 - 8;
   const long tiles = (long)P.ntx * P.nty;

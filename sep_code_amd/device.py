@@ -176,6 +176,10 @@ class BornDevice:
     def set_tuning(self, ty_tile=0, target_blocks=2048):
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
+    def set_x_per_lane(self, cells):
+        """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1 even)."""
+        call("brtm_set_x_per_lane", self.h, int(cells))
+
     def sync(self):
         call("brtm_sync", self.h)
 

@@ -1,0 +1,74 @@
+"""LSRTM time-to-residual: plain CG against learned-preconditioned flexible CG.
+
+    python -m sep_code_amd.lsrtm_bench --n 256 --nt 2500 --weights precond_unet.npy
+
+Both solvers run on the same operator (BF16 source snapshots, resident in
+HBM before the clock starts), from m = 0, until the misfit to the noise-free
+data |L m - d_clean| / |d_clean| <= --tol (see lsrtm.py), and are judged by a
+fresh application of the plain FP32 operator with FP64 norms.  Training is
+excluded from the clock; preconditioner inference is included.  Prints one
+JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import configs, lsrtm
+from .device import BornDevice
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nt", type=int, default=2500)
+    ap.add_argument("--seed", type=int, default=77)
+    ap.add_argument("--niter", type=int, default=400)
+    ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--weights", default="")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--skip-plain", action="store_true")
+    a = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+
+    t0 = time.perf_counter()
+    w = configs.lsrtm(seed=a.seed, n_side=a.n, nt=a.nt)
+    dev = BornDevice(0)
+    op, tprep = lsrtm.setup_operator(dev, w)
+    d, floor, e = lsrtm.observed_data(op, w, keep_noise=True)
+    log(f"{w.name}: {len(w.shots)} shots, nt={a.nt}, prepared in {tprep:.1f}s, "
+        f"noise |e|/|d| = {floor:.4f}, setup {time.perf_counter() - t0:.1f}s")
+    out = {"workload": w.name, "grid": list(w.n), "nt": a.nt, "shots": len(w.shots),
+           "store": w.store, "jimg": w.jimg, "tol": a.tol, "noise_rel": floor}
+    if not a.skip_plain:
+        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="cg",
+                               noise=e, verbose=False)
+        out["cg"] = {"iterations": r.iterations, "converged": r.converged,
+                     "seconds": r.seconds, "judged": r.rel_resid_recomputed,
+                     "hist": r.resid_hist[::5]}
+        log(f"plain CG: {r.iterations} it, {r.seconds:.1f}s, judged {r.rel_resid_recomputed:.4e}")
+    if a.weights:
+        from .unet import UNet
+        net = UNet(dev, batch=a.batch, patch=64, seed=1)
+        net.load(a.weights)
+        dims = lsrtm.model_box(w)[:3]
+        pre = net.precond(dims, overlap=8)
+        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="fpcg",
+                               noise=e, precond=pre)
+        out["fpcg"] = {"iterations": r.iterations, "converged": r.converged,
+                       "seconds": r.seconds, "judged": r.rel_resid_recomputed,
+                       "fallbacks": r.fallbacks, "hist": r.resid_hist[::5]}
+        log(f"learned FPCG: {r.iterations} it, {r.seconds:.1f}s, "
+            f"judged {r.rel_resid_recomputed:.4e}, fallbacks {r.fallbacks}")
+        if "cg" in out and out["cg"]["converged"] and r.converged:
+            out["speedup"] = out["cg"]["seconds"] / r.seconds
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -7,7 +7,9 @@ calibration kernel whose FETCH_SIZE / WRITE_SIZE per byte scale the other
 kernels' counters), then --reps launches each of the stencil variants
   VL plain step, VL + fused imaging (FP32 snapshot), VL + fused Born source
   (BF16 snapshot), VL + fused imaging (BF16 snapshot)
-and prints their HIP-event mean durations and algorithmic bandwidths.
+and prints their HIP-event mean durations and algorithmic bandwidths, for each
+cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
+8-byte accesses; names get an "_x2" suffix).
 """
 from __future__ import annotations
 
@@ -30,6 +32,7 @@ def main():
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--ty", type=int, default=0)
     ap.add_argument("--variants", default="all")
+    ap.add_argument("--xl", default="1,2")
     a = ap.parse_args()
     n = a.n
     dev = BornDevice(0)
@@ -40,11 +43,14 @@ def main():
     nbytes = a.copy_gib * 2 ** 30
     ms = dev.copy_bench(nbytes, a.reps)
     out["copy"] = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / ms / 1e6}
-    for name, form, act, fmt, B in VARIANTS:
-        if a.variants != "all" and name not in a.variants.split(","):
-            continue
-        ms = dev.time_step(form, act, fmt, a.reps)
-        out[name] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6, "gpts": n ** 3 / ms / 1e6}
+    for xl in [int(v) for v in a.xl.split(",")]:
+        dev.set_x_per_lane(xl)
+        for name, form, act, fmt, B in VARIANTS:
+            if a.variants != "all" and name not in a.variants.split(","):
+                continue
+            ms = dev.time_step(form, act, fmt, a.reps)
+            key = name if xl == 1 else name + "_x2"
+            out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6, "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()
 

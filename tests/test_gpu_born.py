@@ -67,6 +67,40 @@ def test_step_parity(dev, n, form, ty):
     dev.set_tuning(0, 2048)
 
 
+@pytest.mark.parametrize("n", [(70, 36, 40), (130, 20, 24), (64, 64, 20), (71, 30, 24)])
+@pytest.mark.parametrize("ty", [8, 16, 32])
+def test_step_parity_x2(dev, n, ty):
+    """Forward-form step with two cells per lane (128-wide tiles, 8-byte
+    accesses): oracle parity and agreement with the one-cell kernel.  n1 = 71
+    is odd, where the library keeps the one-cell kernel."""
+    announce(f"x2 step parity n={n} ty={ty}")
+    r = np.random.default_rng(12)
+    d = (10.0, 11.0, 12.0)
+    V = velocity(r, n, d)
+    dev.set_tuning(ty, 2048)
+    dev.set_x_per_lane(1)
+    dev.create_space(d, 7.0, 0.03, n)
+    dev.set_velocity(V)
+    P = O.Problem(n, d, V, 7.0, 0.03)
+    p1, p0 = random_field(r, n), random_field(r, n)
+    g1, g1o = dev.propagate(0, 60, p1, p0)
+    dev.set_x_per_lane(2)
+    try:
+        c2, _ = dev.propagate(0, 1, p1, p0)
+        g2, g2o = dev.propagate(0, 60, p1, p0)
+    finally:
+        dev.set_x_per_lane(1)
+        dev.set_tuning(0, 2048)
+    o1, _ = P.propagate(0, 1, p1.ravel(), p0.ravel())
+    o_cur, o_old = P.propagate(0, 60, p1.ravel(), p0.ravel())
+    assert rel(c2, o1) < 1e-6
+    assert rel(g2, o_cur) < 1e-5 and rel(g2o, o_old) < 1e-5
+    assert rel(g2, g1) < 1e-6 and rel(g2o, g1o) < 1e-6
+    U = np.zeros(g2.shape, bool)
+    U[4:-4, 4:-4, 4:-4] = True
+    assert np.all(g2[~U] == 0)
+
+
 def make_shot(dev, n=(48, 44, 40), nt=120, jt=1, jimg=1, seed=3, cimg=1e-3, bc_a=8.0,
               bc_b=0.02, d=(10.0, 10.0, 10.0), rec_step=3):
     """Same shot on the device and in the oracle."""
@@ -126,6 +160,37 @@ def test_born_forward_parity(dev, store, direct):
     snaps, _ = P.source_prop()
     Do = P.born_forward(m.ravel(), snaps)
     assert rel(D, Do) < 1e-5
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16", "f16"])
+def test_actions_x2(dev, store):
+    """Born modelling (fused Born source) and the adjoint (fused imaging) on the
+    two-cells-per-lane kernel agree with the one-cell kernel, and with the
+    oracle for FP32 snapshots."""
+    announce(f"x2 born/adjoint store={store}")
+    P, r = make_shot(dev, n=(50, 44, 40), jimg=2)
+    m = random_field(r, P.n)
+    snaps, _ = P.source_prop()
+    Do = P.born_forward(m.ravel(), snaps)
+    out = {}
+    for xl in (1, 2):
+        dev.set_x_per_lane(xl)
+        try:
+            dev.source_prop(1, P.nt, 2, store=store)
+            dev.set_model(m)
+            dev.rtm_forward(1, P.nt, 2)
+            D = dev.get_receivers()
+            dev.set_receivers(P.rlocs, Do)
+            dev.zero_image()
+            dev.rtm_adjoint(1, P.nt, 2)
+            out[xl] = (D, dev.get_image())
+        finally:
+            dev.set_x_per_lane(1)
+    assert rel(out[2][0], out[1][0]) < 1e-6
+    assert rel(out[2][1], out[1][1]) < 1e-6
+    if store == "f32":
+        assert rel(out[2][0], Do) < 1e-5
+        assert rel(out[2][1], P.rtm_adjoint(Do, snaps)) < 1e-4
 
 
 @pytest.mark.parametrize("store,bits", [("bf16", 8), ("f16", 11)])

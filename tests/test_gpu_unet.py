@@ -107,15 +107,26 @@ def test_apply_covers_volume_and_is_scale_equivariant(net):
     net.apply(gv, zv, dims, overlap=8)
     z = zv.numpy().reshape(g.shape)
     assert np.isfinite(z).all() and np.count_nonzero(z) == z.size
-    gv2 = DeviceVector.from_array(net.dev, 3.0 * g)
-    zv2 = DeviceVector(net.dev, g.size)
-    net.apply(gv2, zv2, dims, overlap=8)
-    np.testing.assert_allclose(zv2.numpy(), 3.0 * z.ravel(), rtol=1e-4, atol=1e-4)
+    # z = rms(g) * net(g / rms(g)): a power-of-two factor passes through the
+    # FP64 rms and the FP32 scalings exactly, so the BF16 network sees the
+    # same input bits; another factor changes last bits of g / rms, which the
+    # BF16 input rounding can amplify to one BF16 ulp (2^-8) per value.
+    for f, tol in ((4.0, 1e-6), (3.0, 1e-2)):
+        gv2 = DeviceVector.from_array(net.dev, f * g)
+        zv2 = DeviceVector(net.dev, g.size)
+        net.apply(gv2, zv2, dims, overlap=8)
+        z2 = zv2.numpy().astype(np.float64)
+        err = np.linalg.norm(z2 - f * z.ravel()) / np.linalg.norm(f * z.ravel())
+        assert err < tol, (f, err)
     # the tile with patch origin (40, 40, 40) equals rms * net(patch / rms)
-    # on its central 48^3 (clipped to the volume), zeros outside the volume
-    rms = np.sqrt((g.astype(np.float64) ** 2).mean())
+    # on its central 48^3 (clipped to the volume), zeros outside the volume;
+    # the scalings are the library's FP32 ones so the BF16 inputs agree
+    rms = np.sqrt(float(np.dot(g.ravel().astype(np.float64), g.ravel().astype(np.float64)))
+                  / g.size)
     patch = np.zeros((64, 64, 64), np.float32)
     src = g[40:104, 40:104, 40:104]
-    patch[:src.shape[0], :src.shape[1], :src.shape[2]] = src / rms
-    y = net.forward(patch[None])[0] * rms
-    np.testing.assert_allclose(z[48:70, 48:90, 48:96], y[8:30, 8:50, 8:56], rtol=1e-4, atol=1e-4)
+    patch[:src.shape[0], :src.shape[1], :src.shape[2]] = src * np.float32(1.0 / rms)
+    y = net.forward(patch[None])[0] * np.float32(rms)
+    got, want = z[48:70, 48:90, 48:96], y[8:30, 8:50, 8:56]
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-2
+    np.testing.assert_allclose(got, want, rtol=2e-2, atol=2e-2 * np.abs(want).max())
