@@ -97,6 +97,11 @@ int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data)
  * vectors: err[0] = |<Lm,d>-<m,L'd>|/|<Lm,d>| for add = false, err[1] for
  * add = true.  Returns BRTM_OK; the caller compares against 1e-4. */
 int brtm_op_dot_test(brtm_op *op, unsigned long long seed, double err[2]);
+/* Source illumination on the model grid: out = mute . E^T sum_shots sum_k
+ * s_k^2 over the stored source snapshots (snapshot stores only; no
+ * counterpart in the reference, which has no preconditioner).  Used to
+ * balance image amplitudes for preconditioning. */
+int brtm_op_illumination(brtm_op *op, brtm_vec *out);
 
 /* ---- CG (simple_solver + cgstep) -------------------------------------- */
 /* Preconditioner: z = P(g) on device model vectors; return 0 on success. */
@@ -114,7 +119,18 @@ typedef struct {
    * noise-free data (|L m - d_clean| / |d_clean|).  The iteration itself
    * still minimises |L m - d|. */
   const brtm_vec *offset;
+  /* Steps kept for the search (0 or 1: cgstep's plane search over the new
+   * direction and the previous step, Born/cgstep.cpp:28-53).  h >= 2
+   * (at most BRTM_CG_MAX_HISTORY): flexible CG with truncated history --
+   * each step minimises |rr + L(a z + sum_i b_i s_i)| over the new
+   * direction z and the last h steps s_i, an (h+1)x(h+1) FP64 Gram solve.
+   * With a fixed SPD preconditioner this is CG again; with a preconditioner
+   * that changes between iterations (a network) it keeps the search
+   * directions L-conjugate to the last h steps. */
+  int history;
 } brtm_cg_params;
+
+#define BRTM_CG_MAX_HISTORY 8
 
 typedef struct {
   int iterations;           /* steps taken                                      */

@@ -60,6 +60,8 @@ const char *brtm_version(void);
 /* Replaces setup_cuda (Born/gpu_funcs_3d.cu:10-45): bind a context to one
  * GPU (no peer access: one GPU holds the domain). */
 int brtm_create(int device, brtm_ctx **out);
+/* Vectors, operators and networks created on a context (born_op.h,
+ * born_unet.h) hold it: destroy them before the context. */
 int brtm_destroy(brtm_ctx *ctx);
 
 /* Replaces create_gpu_space (Born/gpu_funcs_3d.cu:777-803): grid spacing,
@@ -167,8 +169,10 @@ int brtm_kernel_timing_result(brtm_ctx *ctx, double *total_ms, int *count);
 int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 
 /* Cells per lane along x for VL-form steps: 1 (64-wide tiles, 4-byte
- * accesses) or 2 (128-wide tiles, 8-byte accesses; used only when n1 is
- * even, other grids keep the 1-cell kernel).  Results are identical. */
+ * accesses), 2 (128-wide tiles, 8-byte accesses; n1 even) or 4 (256-wide
+ * tiles, 16-byte accesses; n1 % 4 == 0).  A grid that does not allow the
+ * width asked for uses the widest one it allows.  Results agree to FP32
+ * rounding. */
 int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes

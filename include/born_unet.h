@@ -57,7 +57,10 @@ int brtm_unet_apply(brtm_unet *net, const brtm_vec *g, brtm_vec *z, int m1, int 
                     int overlap);
 /* Model grid and overlap used by brtm_unet_precond. */
 int brtm_unet_set_grid(brtm_unet *net, int m1, int m2, int m3, int overlap);
-/* brtm_precond_fn for brtm_cg_solve: user = brtm_unet*. */
+/* brtm_precond_fn for brtm_cg_solve: user = brtm_unet*.  The solver's
+ * gradient g = L^T(L m - d) is minus the RTM image of the data residual, so
+ * the network is applied in its training orientation:
+ * z = -s net(-g / s), s = rms(g). */
 int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z);
 
 #ifdef __cplusplus

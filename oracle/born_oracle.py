@@ -349,6 +349,51 @@ def cgstep_solve(A, d, niter):
     return m, res
 
 
+def fcg_solve(A, d, niter, history, precond=None):
+    """Flexible CG with truncated history on min |A m - d| from m = 0, in
+    FP64 with dense A (or a pair of callables (forward, adjoint)).
+
+    Each iteration takes the gradient g = A^T rr, the direction z = P(g, it)
+    (g itself when P is None or when z.g <= 0), and minimises
+    |rr + a A z + sum_i b_i S_i| over a and the coefficients of the last
+    `history` steps (S_i their data-space images).  history = 1 is cgstep's
+    plane search (Born/cgstep.cpp:28-53).  Returns (m, [rr.rr after each
+    iteration], [directions used])."""
+    if callable(A) or isinstance(A, tuple):
+        fwd, adj = A
+    else:
+        Ad = np.asarray(A, np.float64)
+        fwd, adj = (lambda x: Ad @ x), (lambda y: Ad.T @ y)
+    rr = -np.asarray(d, np.float64).ravel().copy()
+    m = None
+    hs, hS, res, dirs = [], [], [], []
+    for it in range(niter):
+        g = adj(rr)
+        if m is None:
+            m = np.zeros_like(g)
+        z = g
+        if precond is not None:
+            zz = np.asarray(precond(g, it), np.float64)
+            if np.isfinite(zz).all() and zz @ g > 0:
+                z = zz
+        G = fwd(z)
+        B = [G] + hS
+        M = np.array([[bi @ bj for bj in B] for bi in B])
+        rhs = -np.array([bi @ rr for bi in B])
+        if M[0, 0] < 1e-20:
+            break
+        x = np.linalg.solve(M, rhs)
+        s = x[0] * z + sum(xi * si for xi, si in zip(x[1:], hs))
+        S = x[0] * G + sum(xi * Si for xi, Si in zip(x[1:], hS))
+        m = m + s
+        rr = rr + S
+        hs = ([s] + hs)[:history]
+        hS = ([S] + hS)[:history]
+        res.append(rr @ rr)
+        dirs.append(z)
+    return m, res, dirs
+
+
 def source_geometry(n1, n2, fz, fx, fy, wavelet_deriv, nts):
     """Born/source_func_3d.cpp:100-133: 9 points on the plane iz-1 around
     (ix, iy) with weights exp(-(dz^2+dx^2+dy^2)); trace sample t+4 holds

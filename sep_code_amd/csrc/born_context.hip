@@ -56,6 +56,19 @@ static int fmt_of_store(int kind) {
 
 static size_t fmt_bytes(int fmt) { return fmt == FMT_F32 ? 4 : 2; }
 
+int illumination(brtm_ctx *c, float *acc) {
+  Store &S = cur_store(c);
+  const int sfmt = fmt_of_store(S.kind);
+  BRTM_REQUIRE(sfmt != FMT_NONE && S.count > 0, BRTM_ESTATE,
+               "illumination needs a snapshot store (f32, bf16 or f16) for the shot");
+  for (int k = 0; k < S.count; ++k) {
+    const void *snap = S.data.as<char>() + fmt_bytes(sfmt) * (size_t)c->ncell * k;
+    const float *ss = sfmt == FMT_F32 ? nullptr : S.scale.as<float>() + (size_t)c->n3 * k;
+    BRTM_HIP(launch_square_accumulate(sfmt, snap, ss, acc, c->n1 * c->n2, c->n3, c->stream));
+  }
+  return BRTM_OK;
+}
+
 // Host-to-device copy ordered on the library stream: work already queued
 // may still read the destination (the stream is non-blocking, so a plain
 // hipMemcpy on the null stream would not wait for it), and the host buffer
@@ -88,14 +101,19 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   // Tile rows / workgroup target per variant (measured on MI355X at 512^3):
   // plain steps prefer tall tiles, 16-bit snapshot variants short ones.
   int ty = c->ty_tile, tb = c->target_blocks;
-  const bool x2 = c->vec2 && form == FORM_VL && (c->n1 % 2) == 0;
+  // cells per lane: 4 or 2 where asked for and n1 allows, else 1
+  int xl = 1;
+  if (form == FORM_VL && c->x_per_lane == 4 && c->n1 % 4 == 0) xl = 4;
+  else if (form == FORM_VL && c->x_per_lane >= 2 && c->n1 % 2 == 0) xl = 2;
   if (ty == 0) {
-    if (x2) { ty = 16; tb = 2048; }
+    if (xl == 4) { ty = 8; tb = 2048; }
+    else if (xl == 2) { ty = 16; tb = 2048; }
     else if (act == ACT_NONE) { ty = 32; tb = 1024; }
     else if (fmt == FMT_F32) { ty = 16; tb = 8192; }
     else { ty = 8; tb = 8192; }
   }
-  plan_step(P, ty, tb, x2 ? 128 : 64);
+  if (xl == 4 && ty == 32) ty = 16;  // the 4-cell kernel's tiles are 8 or 16 rows
+  plan_step(P, ty, tb, 64 * xl);
   const bool timed =
       c->ktime_on && form == c->ktime_form && act == c->ktime_act && fmt == c->ktime_fmt;
   if (timed) {
@@ -108,7 +126,9 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  if (x2)
+  if (xl == 4)
+    BRTM_HIP(launch_step_x4(act, fmt, P, ty, c->stream));
+  else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
     BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream));
@@ -774,8 +794,9 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
 
 int brtm_set_x_per_lane(brtm_ctx *c, int cells) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
-  BRTM_REQUIRE(cells == 1 || cells == 2, BRTM_EINVAL, "cells per lane must be 1 or 2");
-  c->vec2 = cells == 2;
+  BRTM_REQUIRE(cells == 1 || cells == 2 || cells == 4, BRTM_EINVAL,
+               "cells per lane must be 1, 2 or 4");
+  c->x_per_lane = cells;
   return BRTM_OK;
 }
 

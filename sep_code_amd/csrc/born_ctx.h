@@ -71,7 +71,7 @@ struct brtm_ctx {
   std::vector<float> wtab;
   float cimg = 1e-6f;
   int ty_tile = 0, target_blocks = 2048;  // ty_tile 0: chosen per step variant
-  bool vec2 = false;                       // VL steps with two x per lane (n1 even)
+  int x_per_lane = 1;                      // VL steps: 1, 2 (n1 even) or 4 (n1 % 4 == 0)
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)
@@ -111,6 +111,8 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int store, int ckpt, floa
 int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate);
 int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg);
 int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out);
+// acc += sum over the current shot's stored snapshots of s^2 (compute grid).
+int illumination(brtm_ctx *c, float *acc);
 Store &cur_store(brtm_ctx *c);
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img_or_null, const float *m_or_null);

@@ -41,8 +41,10 @@ struct StepParams {
 hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with two consecutive x per lane (128-wide tiles, n1 even).
 hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// The VL step with four consecutive x per lane (256-wide tiles, n1 % 4 == 0).
+hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // Tile / chunk geometry chosen for a grid (fills ntx..zc of P); xw is the
-// tile width in x (64 for launch_step, 128 for launch_step_x2).
+// tile width in x (64, 128 or 256 for the 1-, 2- and 4-cell kernels).
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw = 64);
 
 // p[loc_j] += scale * sum_k sinc8[k] tr(j, id+k), tr(j,t) = tr[j*sp + t*st].
@@ -84,6 +86,10 @@ hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int 
 // Decode one snapshot to FP32 (tests, diagnostics).
 hipError_t launch_decode(int fmt, const void *src, const float *sscale, float *dst, int n12,
                          int n3, hipStream_t st);
+
+// acc += s^2 over one snapshot (illumination).
+hipError_t launch_square_accumulate(int fmt, const void *src, const float *sscale, float *acc,
+                                   int n12, int n3, hipStream_t st);
 
 // FP64 reductions / vector kernels.
 constexpr int kDotBlocks = 1024;

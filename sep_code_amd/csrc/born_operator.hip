@@ -360,6 +360,27 @@ int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data)
   return BRTM_OK;
 }
 
+int brtm_op_illumination(brtm_op *op, brtm_vec *out) {
+  BRTM_REQUIRE(op, BRTM_EINVAL, "NULL operator");
+  BRTM_REQUIRE(op->prepared, BRTM_ESTATE, "brtm_op_prepare has not been called");
+  OP_TRY(check_vec(out, op->model_size(), "out"));
+  brtm_ctx *c = op->ctx;
+  const brtm_op_desc &d = op->d;
+  BRTM_HIP(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  BRTM_HIP(hipMemsetAsync(out->buf.ptr, 0, sizeof(float) * (size_t)out->n, st));
+  float *acc = op->work.as<float>();
+  BRTM_HIP(hipMemsetAsync(acc, 0, sizeof(float) * (size_t)c->ncell, st));
+  for (int s = 0; s < d.nshots; ++s) {
+    OP_TRY(brtm_select_shot(c, s));
+    OP_TRY(illumination(c, acc));
+  }
+  BRTM_HIP(launch_embed(1, out->p(), acc, d.m1, d.m2, d.m3, c->n1, c->n2, c->n3, d.o1, d.o2,
+                        d.o3, st));
+  if (op->d_mute.ptr) BRTM_HIP(launch_mul(out->p(), op->d_mute.as<float>(), out->n, st));
+  return BRTM_OK;
+}
+
 int brtm_op_dot_test(brtm_op *op, unsigned long long seed, double err[2]) {
   BRTM_REQUIRE(op && err, BRTM_EINVAL, "NULL argument");
   brtm_ctx *c = op->ctx;
@@ -394,6 +415,55 @@ int brtm_op_dot_test(brtm_op *op, unsigned long long seed, double err[2]) {
 }
 
 // ------------------------------------------------------------------- CG
+namespace {
+
+// Solve the K x K symmetric system M x = b (FP64) by Cholesky, skipping
+// basis vectors whose pivot falls below tol times their own norm (nearly
+// dependent on the ones before them): their coefficient is 0.  Returns the
+// number of basis vectors kept (0 when the first one has no length).
+int gram_solve(int K, const double *M, const double *b, double *x, double tol = 1e-9) {
+  double Lm[BRTM_CG_MAX_HISTORY + 1][BRTM_CG_MAX_HISTORY + 1] = {};
+  bool keep[BRTM_CG_MAX_HISTORY + 1] = {};
+  int kept = 0;
+  for (int i = 0; i < K; ++i) {
+    double d = M[i * K + i];
+    for (int k = 0; k < i; ++k)
+      if (keep[k]) d -= Lm[i][k] * Lm[i][k];
+    if (!(d > tol * M[i * K + i]) || !(M[i * K + i] > 0)) {
+      keep[i] = false;
+      continue;
+    }
+    keep[i] = true;
+    ++kept;
+    Lm[i][i] = std::sqrt(d);
+    for (int j = i + 1; j < K; ++j) {
+      double v = M[j * K + i];
+      for (int k = 0; k < i; ++k)
+        if (keep[k]) v -= Lm[j][k] * Lm[i][k];
+      Lm[j][i] = v / Lm[i][i];
+    }
+  }
+  double y[BRTM_CG_MAX_HISTORY + 1] = {};
+  for (int i = 0; i < K; ++i) {
+    if (!keep[i]) continue;
+    double v = b[i];
+    for (int k = 0; k < i; ++k)
+      if (keep[k]) v -= Lm[i][k] * y[k];
+    y[i] = v / Lm[i][i];
+  }
+  for (int i = K - 1; i >= 0; --i) {
+    x[i] = 0;
+    if (!keep[i]) continue;
+    double v = y[i];
+    for (int k = i + 1; k < K; ++k)
+      if (keep[k]) v -= Lm[k][i] * x[k];
+    x[i] = v / Lm[i][i];
+  }
+  return kept;
+}
+
+}  // namespace
+
 int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
                   const brtm_cg_params *prm, double *resid_hist, double *time_hist,
                   brtm_cg_stats *stats) {
@@ -403,15 +473,29 @@ int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
   brtm_ctx *c = op->ctx;
   brtm_cg_stats S{};
   const long nm = op->model_size(), nd = op->data_size();
+  BRTM_REQUIRE(prm->history >= 0 && prm->history <= BRTM_CG_MAX_HISTORY, BRTM_EINVAL,
+               "history must be in [0, BRTM_CG_MAX_HISTORY]");
+  const int H = prm->history >= 2 ? prm->history : 0;  // 0: cgstep
   brtm_vec *g = nullptr, *z = nullptr, *s = nullptr, *rr = nullptr, *gg = nullptr, *ss = nullptr;
+  // flexible CG: the last H steps in model (hs) and data (hS) space, newest first
+  std::vector<brtm_vec *> hs(H, nullptr), hS(H, nullptr);
   int rc = brtm_vec_create(c, nm, &g);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, nm, &s);
   if (rc == BRTM_OK && prm->precond) rc = brtm_vec_create(c, nm, &z);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &rr);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &gg);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &ss);
+  for (int i = 0; i < H && rc == BRTM_OK; ++i) {
+    rc = brtm_vec_create(c, nm, &hs[i]);
+    if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &hS[i]);
+  }
+  int nh = 0;  // steps held
   auto cleanup = [&]() {
     for (brtm_vec *v : {g, z, s, rr, gg, ss}) brtm_vec_destroy(v);
+    for (int i = 0; i < H; ++i) {
+      brtm_vec_destroy(hs[i]);
+      brtm_vec_destroy(hS[i]);
+    }
   };
   if (rc != BRTM_OK) {
     cleanup();
@@ -490,42 +574,88 @@ int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
     }
     rc = brtm_op_forward(op, 0, dir, gg);
     if (rc != BRTM_OK) break;
-    // cgstep::step_it (Born/cgstep.cpp:14-58)
     double alfa, beta;
-    double gg_gg = 0;
-    rc = brtm_vec_dot(gg, gg, &gg_gg);
-    if (rc != BRTM_OK) break;
-    if (first) {
-      if (std::fabs(gg_gg) < small) {
+    if (H) {
+      // Flexible CG: minimise |rr + a gg + sum_i b_i hS_i| over (a, b).
+      const int K = nh + 1;
+      const brtm_vec *B[BRTM_CG_MAX_HISTORY + 1];
+      B[0] = gg;
+      for (int i = 0; i < nh; ++i) B[i + 1] = hS[i];
+      double M[(BRTM_CG_MAX_HISTORY + 1) * (BRTM_CG_MAX_HISTORY + 1)], bv[BRTM_CG_MAX_HISTORY + 1],
+          x[BRTM_CG_MAX_HISTORY + 1];
+      for (int a = 0; a < K && rc == BRTM_OK; ++a) {
+        for (int b = 0; b <= a && rc == BRTM_OK; ++b) {
+          rc = brtm_vec_dot(B[a], B[b], &M[a * K + b]);
+          M[b * K + a] = M[a * K + b];
+        }
+        if (rc == BRTM_OK) rc = brtm_vec_dot(B[a], rr, &bv[a]);
+        bv[a] = -bv[a];
+      }
+      if (rc != BRTM_OK) break;
+      if (std::fabs(M[0]) < small || gram_solve(K, M, bv, x) == 0) {
         S.breakdown = 1;
         break;
       }
-      double gg_rr = 0;
-      rc = brtm_vec_dot(gg, rr, &gg_rr);
+      // new step into the scratch pair (s, ss)
+      rc = brtm_vec_copy(s, dir);
+      if (rc == BRTM_OK) rc = brtm_vec_scale(s, x[0]);
+      if (rc == BRTM_OK) rc = brtm_vec_copy(ss, gg);
+      if (rc == BRTM_OK) rc = brtm_vec_scale(ss, x[0]);
+      for (int i = 0; i < nh && rc == BRTM_OK; ++i) {
+        if (x[i + 1] == 0) continue;
+        rc = brtm_vec_scale_add(s, 1.0, hs[i], x[i + 1]);
+        if (rc == BRTM_OK) rc = brtm_vec_scale_add(ss, 1.0, hS[i], x[i + 1]);
+      }
+      if (rc == BRTM_OK) rc = brtm_vec_add(model, s);
+      if (rc == BRTM_OK) rc = brtm_vec_add(rr, ss);
       if (rc != BRTM_OK) break;
-      beta = 0;
-      alfa = -gg_rr / gg_gg;
+      // push: the new step becomes hs[0]; the dropped (or unused) slot
+      // becomes the next scratch pair
+      const int slot = nh < H ? nh : H - 1;
+      std::swap(s, hs[slot]);
+      std::swap(ss, hS[slot]);
+      std::rotate(hs.begin(), hs.begin() + slot, hs.begin() + slot + 1);
+      std::rotate(hS.begin(), hS.begin() + slot, hS.begin() + slot + 1);
+      nh = std::min(nh + 1, H);
+      alfa = x[0];
+      beta = nh > 1 ? x[1] : 0.0;
     } else {
-      double ss_ss = 0, gg_ss = 0, gg_rr = 0, ss_rr = 0;
-      rc = brtm_vec_dot(ss, ss, &ss_ss);
-      if (rc == BRTM_OK) rc = brtm_vec_dot(gg, ss, &gg_ss);
-      if (rc == BRTM_OK) rc = brtm_vec_dot(gg, rr, &gg_rr);
-      if (rc == BRTM_OK) rc = brtm_vec_dot(ss, rr, &ss_rr);
+      // cgstep::step_it (Born/cgstep.cpp:14-58)
+      double gg_gg = 0;
+      rc = brtm_vec_dot(gg, gg, &gg_gg);
       if (rc != BRTM_OK) break;
-      if (std::fabs(gg_gg) < small || std::fabs(ss_ss) < small) {
-        S.breakdown = 1;
-        break;
+      if (first) {
+        if (std::fabs(gg_gg) < small) {
+          S.breakdown = 1;
+          break;
+        }
+        double gg_rr = 0;
+        rc = brtm_vec_dot(gg, rr, &gg_rr);
+        if (rc != BRTM_OK) break;
+        beta = 0;
+        alfa = -gg_rr / gg_gg;
+      } else {
+        double ss_ss = 0, gg_ss = 0, gg_rr = 0, ss_rr = 0;
+        rc = brtm_vec_dot(ss, ss, &ss_ss);
+        if (rc == BRTM_OK) rc = brtm_vec_dot(gg, ss, &gg_ss);
+        if (rc == BRTM_OK) rc = brtm_vec_dot(gg, rr, &gg_rr);
+        if (rc == BRTM_OK) rc = brtm_vec_dot(ss, rr, &ss_rr);
+        if (rc != BRTM_OK) break;
+        if (std::fabs(gg_gg) < small || std::fabs(ss_ss) < small) {
+          S.breakdown = 1;
+          break;
+        }
+        const double determ = gg_gg * ss_ss * (1.0 - gg_ss / gg_gg * gg_ss / ss_ss);
+        gg_rr = -gg_rr;
+        ss_rr = -ss_rr;
+        alfa = (ss_ss * gg_rr - gg_ss * ss_rr) / determ;
+        beta = (-gg_ss * gg_rr + gg_gg * ss_rr) / determ;
       }
-      const double determ = gg_gg * ss_ss * (1.0 - gg_ss / gg_gg * gg_ss / ss_ss);
-      gg_rr = -gg_rr;
-      ss_rr = -ss_rr;
-      alfa = (ss_ss * gg_rr - gg_ss * ss_rr) / determ;
-      beta = (-gg_ss * gg_rr + gg_gg * ss_rr) / determ;
+      rc = brtm_vec_scale_add(s, beta, dir, alfa);       // s  = beta s  + alfa g
+      if (rc == BRTM_OK) rc = brtm_vec_scale_add(ss, beta, gg, alfa);  // ss = beta ss + alfa gg
+      if (rc == BRTM_OK) rc = brtm_vec_add(model, s);                  // m += s
+      if (rc == BRTM_OK) rc = brtm_vec_add(rr, ss);                    // rr += ss
     }
-    rc = brtm_vec_scale_add(s, beta, dir, alfa);       // s  = beta s  + alfa g
-    if (rc == BRTM_OK) rc = brtm_vec_scale_add(ss, beta, gg, alfa);  // ss = beta ss + alfa gg
-    if (rc == BRTM_OK) rc = brtm_vec_add(model, s);                  // m += s
-    if (rc == BRTM_OK) rc = brtm_vec_add(rr, ss);                    // rr += ss
     double rr_rr = 0;
     if (rc == BRTM_OK) rc = brtm_vec_dot(rr, rr, &rr_rr);
     if (rc != BRTM_OK) break;

@@ -420,8 +420,9 @@ int brtm_unet_extract(brtm_unet *u, const float *vol, int m1, int m2, int m3, in
   return BRTM_OK;
 }
 
-int brtm_unet_apply(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3,
-                    int ov) {
+// z = sign . rms(g) . net(sign . g / rms(g)), patch by patch.
+static int apply_signed(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3, int ov,
+                 float sign) {
   BRTM_REQUIRE(u && g && z, BRTM_EINVAL, "NULL argument");
   BRTM_REQUIRE((long)m1 * m2 * m3 == brtm_vec_size(g) && brtm_vec_size(z) == brtm_vec_size(g),
                BRTM_EINVAL, "vector size does not match the model grid");
@@ -450,7 +451,8 @@ int brtm_unet_apply(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2
     const int nb = std::min(u->B, np - p0);
     for (int k = 0; k < nb; ++k) {
       const int *o = &orig[3 * (p0 + k)];
-      k_extract<<<1024, 256, 0, st>>>(gp, m1, m2, m3, o[0], o[1], o[2], (float)(1.0 / rms),
+      k_extract<<<1024, 256, 0, st>>>(gp, m1, m2, m3, o[0], o[1], o[2],
+                                      sign * (float)(1.0 / rms),
                                       u->xpatch.as<float>() + k * u->P3, u->P);
     }
     BRTM_HIP(hipGetLastError());
@@ -458,11 +460,16 @@ int brtm_unet_apply(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2
     for (int k = 0; k < nb; ++k) {
       const int *o = &orig[3 * (p0 + k)];
       k_insert<<<1024, 256, 0, st>>>(u->out.as<float>() + k * u->P3, u->P, ov, zp, m1, m2, m3,
-                                     o[0], o[1], o[2], (float)rms);
+                                     o[0], o[1], o[2], sign * (float)rms);
     }
     BRTM_HIP(hipGetLastError());
   }
   return BRTM_OK;
+}
+
+int brtm_unet_apply(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3,
+                    int ov) {
+  return apply_signed(u, g, z, m1, m2, m3, ov, 1.f);
 }
 
 int brtm_unet_set_grid(brtm_unet *u, int m1, int m2, int m3, int overlap) {
@@ -481,7 +488,10 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
     set_error("brtm_unet_precond: call brtm_unet_set_grid first");
     return BRTM_ESTATE;
   }
-  return brtm_unet_apply(u, g, z, u->gm1, u->gm2, u->gm3, u->gov);
+  // The solver's gradient is g = L^T (L m - d) = -(RTM image of the data
+  // residual); the network maps RTM images to LSRTM images, so it is given -g
+  // and its output is negated back: z = -rms . net(-g / rms).
+  return apply_signed(u, g, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f);
 }
 
 }  // extern "C"

@@ -35,8 +35,16 @@ class BornDevice:
         context; it is closed before the context is destroyed."""
         self._children.add(obj)
 
+    def alive(self) -> bool:
+        """False once the context is destroyed.  Children check it before
+        their own destroy call: when the garbage collector finalises a
+        context together with its children (weak references cleared first),
+        the order is arbitrary, and a child left over after the context only
+        drops its handle (its device memory went with the process)."""
+        return getattr(self, "h", None) is not None and bool(self.h.value)
+
     def close(self):
-        if getattr(self, "h", None) is not None and self.h.value:
+        if self.alive():
             for ch in list(self._children):
                 ch.close()
             call("brtm_destroy", self.h)
@@ -177,7 +185,8 @@ class BornDevice:
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
     def set_x_per_lane(self, cells):
-        """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1 even)."""
+        """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1
+        even); 4: 256-wide tiles with 16-byte accesses (VL, n1 % 4 == 0)."""
         call("brtm_set_x_per_lane", self.h, int(cells))
 
     def sync(self):

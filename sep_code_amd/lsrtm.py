@@ -85,11 +85,34 @@ def observed_data(op, w, noise_frac=0.01, seed=1234, keep_noise=False):
     return d, floor
 
 
+def illumination_weight(op, eps=1e-2, power=1.0):
+    """Diagonal w = (I + eps max I)^-power of the source illumination I
+    (sum over shots and stored snapshots of s^2, on the model grid) as a
+    device vector.  power = 1 is the classical illumination-compensation
+    preconditioner; power = 1/2 its square root, for split use."""
+    illum = op.illumination()
+    a = illum.numpy().astype(np.float64)
+    illum.close()
+    wv = (a + eps * a.max()) ** -power
+    return DeviceVector.from_array(op.dev, (wv / wv.max()).astype(np.float32))
+
+
+def diagonal_precond(dev, wv):
+    """Preconditioner callback z = w . g (device vectors, no host copies)."""
+    n = wv.n
+
+    def pre(g_h, z_h):
+        DeviceVector.view(dev, z_h, n).copy_from(DeviceVector.view(dev, g_h, n)).mult(wv)
+        return 0
+    return pre
+
+
 def run_lsrtm(dev, w, op, d, niter=60, rel_tol=1e-2, precond=None, name="cg", verbose=False,
-              floor=0.0, prepare_seconds=0.0, noise=None):
+              floor=0.0, prepare_seconds=0.0, noise=None, history=0):
     """Solve and judge.  With noise e given, the residual judged (for the
-    stop and the final check) is the misfit to the noise-free data d - e."""
-    solver = SimpleSolver(cgstep(), d, op, precond=precond, verbose=verbose)
+    stop and the final check) is the misfit to the noise-free data d - e.
+    history >= 2 selects flexible CG with that many previous steps."""
+    solver = SimpleSolver(cgstep(), d, op, precond=precond, verbose=verbose, history=history)
     res = solver.solve(niter, rel_tol=rel_tol, offset=noise)
     # judge: fresh plain-operator residual with FP64 norms
     m = solver.return_model()

@@ -32,6 +32,11 @@ def main():
     ap.add_argument("--weights", default="")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--history", type=int, default=4,
+                    help="steps kept by the flexible CG of the learned run")
+    ap.add_argument("--diag-eps", default="",
+                    help="comma list: also run CG preconditioned by the illumination "
+                         "diagonal (I + eps max I)^-1 for each eps")
     a = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
 
@@ -51,6 +56,16 @@ def main():
                      "seconds": r.seconds, "judged": r.rel_resid_recomputed,
                      "hist": r.resid_hist[::5]}
         log(f"plain CG: {r.iterations} it, {r.seconds:.1f}s, judged {r.rel_resid_recomputed:.4e}")
+    for eps in [float(v) for v in a.diag_eps.split(",") if v]:
+        wv = lsrtm.illumination_weight(op, eps)
+        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="diag",
+                               noise=e, precond=lsrtm.diagonal_precond(dev, wv))
+        out[f"diag_{eps:g}"] = {"iterations": r.iterations, "converged": r.converged,
+                                "seconds": r.seconds, "judged": r.rel_resid_recomputed,
+                                "hist": r.resid_hist[::5]}
+        log(f"illumination-diagonal PCG eps={eps:g}: {r.iterations} it, {r.seconds:.1f}s, "
+            f"judged {r.rel_resid_recomputed:.4e}")
+        wv.close()
     if a.weights:
         from .unet import UNet
         net = UNet(dev, batch=a.batch, patch=64, seed=1)
@@ -58,8 +73,8 @@ def main():
         dims = lsrtm.model_box(w)[:3]
         pre = net.precond(dims, overlap=8)
         r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="fpcg",
-                               noise=e, precond=pre)
-        out["fpcg"] = {"iterations": r.iterations, "converged": r.converged,
+                               noise=e, precond=pre, history=a.history)
+        out["fpcg"] = {"history": a.history, "iterations": r.iterations, "converged": r.converged,
                        "seconds": r.seconds, "judged": r.rel_resid_recomputed,
                        "fallbacks": r.fallbacks, "hist": r.resid_hist[::5]}
         log(f"learned FPCG: {r.iterations} it, {r.seconds:.1f}s, "

@@ -39,7 +39,7 @@ PRECOND_FN = C.CFUNCTYPE(_I, _P, _P, _P)
 
 class CgParams(C.Structure):
     _fields_ = [("niter", _I), ("rel_tol", _D), ("precond", PRECOND_FN), ("precond_user", _P),
-                ("verbose", _I), ("offset", _P)]
+                ("verbose", _I), ("offset", _P), ("history", _I)]
 
 
 class CgStats(C.Structure):
@@ -70,6 +70,7 @@ _SIGS = {
     "brtm_op_forward": [_P, _I, _P, _P],
     "brtm_op_adjoint": [_P, _I, _P, _P],
     "brtm_op_dot_test": [_P, C.c_ulonglong, C.POINTER(_D)],
+    "brtm_op_illumination": [_P, _P],
     "brtm_cg_solve": [_P, _P, _P, C.POINTER(CgParams), _P, _P, C.POINTER(CgStats)],
     "brtm_sinc_table": [_I, _I, _P],
 }
@@ -107,6 +108,17 @@ class DeviceVector:
         dev.adopt(self)
 
     @classmethod
+    def view(cls, dev, handle, n: int):
+        """Non-owning wrapper of a library vector handle (e.g. the g and z a
+        preconditioner callback receives)."""
+        v = cls.__new__(cls)
+        v.dev = dev
+        v.n = int(n)
+        v.h = _P(handle)
+        v._view = True
+        return v
+
+    @classmethod
     def from_array(cls, dev, a):
         a = f32(a).reshape(-1)
         v = cls(dev, a.size)
@@ -114,7 +126,13 @@ class DeviceVector:
         return v
 
     def close(self):
+        if getattr(self, "_view", False):
+            self.h = None
+            return
         if getattr(self, "h", None) is not None and self.h.value:
+            if not self.dev.alive():
+                self.h = None
+                return
             call("brtm_vec_destroy", self.h)
             self.h = None
 
@@ -219,6 +237,9 @@ class RtmZeroOp:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
+            if not self.dev.alive():
+                self.h = None
+                return
             call("brtm_op_destroy", self.h)
             self.h = None
 
@@ -260,6 +281,12 @@ class RtmZeroOp:
         call("brtm_op_adjoint", self.h, int(bool(add)), model.h, data.h)
         return True
 
+    def illumination(self):
+        """Source illumination sum_shots sum_k s_k^2 on the model grid (muted)."""
+        v = DeviceVector(self.dev, self.model_size)
+        call("brtm_op_illumination", self.h, v.h)
+        return v
+
     def dot_test(self, verb=False, seed=12345):
         err = (_D * 2)()
         call("brtm_op_dot_test", self.h, C.c_ulonglong(seed), err)
@@ -288,9 +315,14 @@ class SolveResult:
 class SimpleSolver:
     """simple_solver(step, data, op) with solve(niter) (Born/lin_solver.cpp)."""
 
-    def __init__(self, step, data: DeviceVector, op: RtmZeroOp, precond=None, verbose=False):
+    def __init__(self, step, data: DeviceVector, op: RtmZeroOp, precond=None, verbose=False,
+                 history=0):
+        """history: 0/1 = cgstep's plane search; h >= 2 = flexible CG over the
+        new direction and the last h steps (for preconditioners that change
+        between iterations)."""
         assert isinstance(step, cgstep)
         self.op = op
+        self.history = int(history)
         self.data = data
         self.model = op.model_vector()
         self.verbose = verbose
@@ -317,7 +349,7 @@ class SimpleSolver:
         p = CgParams(int(niter), float(rel_tol),
                      self._precond_cb if self._precond_cb is not None else PRECOND_FN(),
                      self._precond_user, int(self.verbose),
-                     offset.h if offset is not None else None)
+                     offset.h if offset is not None else None, self.history)
         rh = np.zeros(niter + 1)
         th = np.zeros(niter + 1)
         st = CgStats()

@@ -9,7 +9,8 @@ kernels' counters), then --reps launches each of the stencil variants
   (BF16 snapshot), VL + fused imaging (BF16 snapshot)
 and prints their HIP-event mean durations and algorithmic bandwidths, for each
 cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
-8-byte accesses; names get an "_x2" suffix).
+8-byte accesses; 4: 256-wide tiles with 16-byte accesses; names get an
+"_x2" / "_x4" suffix).
 """
 from __future__ import annotations
 
@@ -49,7 +50,7 @@ def main():
             if a.variants != "all" and name not in a.variants.split(","):
                 continue
             ms = dev.time_step(form, act, fmt, a.reps)
-            key = name if xl == 1 else name + "_x2"
+            key = name if xl == 1 else f"{name}_x{xl}"
             out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6, "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()

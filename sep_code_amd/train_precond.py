@@ -2,12 +2,18 @@
 
 Training pairs are made by this library on seeded random models (layered,
 dipping and Gaussian-anomaly velocities with sparse reflectivity, the
-acquisition of the LSRTM workload: 8 shots on a 2 x 4 surface grid):
-  input  = RTM image       L^T d
-  target = LSRTM image     20 iterations of cgstep on |L m - d|
-each normalised by its own RMS (the CG plane search fixes the step scale, so
-only the direction of P(g) matters).  The network is trained with its own
-forward/backward kernels and Adam on random 64^3 patches with x/y flips.
+acquisition of the LSRTM workload: 8 shots on a 2 x 4 surface grid, 1 %
+noise).  One 20-iteration cgstep LSRTM run per model gives
+  input  = RTM image       L^T d                (= -g_0)
+  target = LSRTM image     m_20
+and, from the same run, pairs for the later gradients the preconditioner
+meets inside the solver:
+  input  = -g_k = L^T (d - L m_k)   (RTM image of the data residual)
+  target = m_20 - m_k               (the rest of the 20-iteration update)
+for k in --states (k = 0 is the plain pair).  Each volume is normalised by
+its own RMS (the CG search fixes the step length, so only the direction of
+P(g) matters).  The network is trained with its own forward/backward kernels
+and Adam on random 64^3 patches, with random sign (the map is odd).
 
     python -m sep_code_amd.train_precond --pairs 96 --out weights.npy
 """
@@ -22,21 +28,41 @@ import numpy as np
 
 from . import configs, lsrtm
 from .device import BornDevice
-from .operator import DeviceVector
+from .operator import DeviceVector, SimpleSolver, cgstep
 from .unet import UNet
 
 
-def make_pair(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01):
+def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(0,)):
+    """One LSRTM run on the seeded model; returns ([(input, target)], dims,
+    relative residual after cg_iters)."""
     w = configs.lsrtm(seed=seed, n_side=n_side, nt=nt, anomalies=int(seed % 4))
     op, _ = lsrtm.setup_operator(dev, w)
     d, _ = lsrtm.observed_data(op, w, noise_frac=noise, seed=seed + 99)
-    rtm = op.model_vector()
-    op.adjoint(False, rtm, d)
-    res, m = lsrtm.run_lsrtm(dev, w, op, d, niter=cg_iters, rel_tol=0.0)
-    x, y = rtm.numpy(), m.numpy()
+    nm = op.model_size
+    kept = {}
+    solver = None
+
+    def record(g_h, z_h):
+        # identity preconditioner (z = g: the plain cgstep iterates) that
+        # records g_k and m_k on the way
+        it = len(kept.setdefault("_calls", []))
+        kept["_calls"].append(it)
+        g = DeviceVector.view(dev, g_h, nm)
+        DeviceVector.view(dev, z_h, nm).copy_from(g)
+        if it in states:
+            kept[it] = (g.numpy(), solver.model.numpy())
+        return 0
+
+    solver = SimpleSolver(cgstep(), d, op, precond=record)
+    res = solver.solve(cg_iters)
+    m20 = solver.model.numpy()
+    pairs = [(-kept[k][0], m20 - kept[k][1]) for k in states if k in kept]
     dims = lsrtm.model_box(w)[:3]
+    # rebinding the closure cell breaks the solver <-> callback cycle, so
+    # the device vectors are released now, in order, by reference counting
+    solver = None
     op.close()
-    return x, y, dims, res.rel_resid
+    return pairs, dims, float(res.rel_resid)
 
 
 def normalise(a):
@@ -59,8 +85,9 @@ def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print):
         for k in range(batch):
             xv, yv, dims = vols[rng.integers(len(vols))]
             o = [int(rng.integers(-8, max(-7, dims[a] - P + 9))) for a in range(3)]
-            net.extract(xv, dims, o, 1.0, "x", k)
-            net.extract(yv, dims, o, 1.0, "t", k)
+            sg = 1.0 if rng.random() < 0.5 else -1.0
+            net.extract(xv, dims, o, sg, "x", k)
+            net.extract(yv, dims, o, sg, "t", k)
         loss = net.train_step_device(batch, cur_lr)
         hist.append(loss)
         if it % 100 == 0 or it == steps - 1:
@@ -77,25 +104,33 @@ def main():
     ap.add_argument("--seed0", type=int, default=1000)
     ap.add_argument("--out", default="precond_unet.npy")
     ap.add_argument("--save-pairs", default="")
+    ap.add_argument("--states", default="0,4,9,14",
+                    help="CG iterations whose gradient / remaining update form pairs")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=4)
     a = ap.parse_args()
+    states = tuple(int(v) for v in a.states.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     dev = BornDevice(0)
     pairs = []
     t0 = time.perf_counter()
     for i in range(a.pairs):
-        x, y, dims, rr = make_pair(dev, a.seed0 + i, n_side=a.n, nt=a.nt)
-        pairs.append((x, y, dims))
-        log(f"pair {i + 1}/{a.pairs} residual after 20 CG {rr:.3f} "
+        ps, dims, rr = make_pairs(dev, a.seed0 + i, n_side=a.n, nt=a.nt, states=states)
+        pairs.extend((x, y, dims) for x, y in ps)
+        log(f"model {i + 1}/{a.pairs}: {len(ps)} pairs, residual after 20 CG {rr:.3f} "
             f"({time.perf_counter() - t0:.1f}s)")
     if a.save_pairs:
         np.savez(a.save_pairs, **{f"x{i}": p[0] for i, p in enumerate(pairs)},
                  **{f"y{i}": p[1] for i, p in enumerate(pairs)})
     tgen = time.perf_counter() - t0
-    net, hist = train(dev, pairs, steps=a.steps, log=log)
+    net, hist = train(dev, pairs, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
     net.save(a.out)
-    print(json.dumps({"pairs": a.pairs, "gen_seconds": tgen, "steps": a.steps,
+    print(json.dumps({"models": a.pairs, "pairs": len(pairs), "states": list(states),
+                      "gen_seconds": tgen, "steps": a.steps,
                       "loss_first100": float(np.mean(hist[:100])),
                       "loss_last100": float(np.mean(hist[-100:]))}), flush=True)
+    net.close()
+    dev.close()
 
 
 if __name__ == "__main__":

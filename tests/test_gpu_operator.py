@@ -241,3 +241,81 @@ def test_cg_residual_stop_and_bad_preconditioner_fallback(dev):
     assert bad.fallbacks == 4
     np.testing.assert_allclose(bad.resid_hist, full.resid_hist, rtol=1e-5)
     op.close()
+
+
+@pytest.mark.parametrize("history", [1, 3])
+def test_flexible_cg_matches_oracle(dev, history):
+    """Flexible CG with a preconditioner that changes every iteration (a
+    different positive diagonal each time): device residual history and model
+    against the FP64 oracle's fcg_solve on the same composite operator."""
+    announce(f"flexible cg history={history}")
+    w = small_workload(nt=60, n_side=32)
+    mbox, mu = setup(dev, w)
+    op = RtmZeroOp(dev, w, store="f32", model_box=mbox, mute=mu)
+    op.prepare()
+    orc = oracle_for(w, mbox, mu)
+    r = np.random.default_rng(17)
+    m_true = r.standard_normal(op.model_size)
+    d = orc.forward(m_true)
+    mshape = (mbox[2], mbox[1], mbox[0])
+    P = [r.uniform(0.3, 3.0, op.model_size) for _ in range(6)]
+    calls = []
+
+    def pre(g_h, z_h):
+        it = len(calls)
+        calls.append(it)
+        g = DeviceVector.view(dev, g_h, op.model_size).numpy()
+        DeviceVector.view(dev, z_h, op.model_size).upload(P[it] * g)
+        return 0
+
+    solver = SimpleSolver(cgstep(), op.data_vector(d), op, precond=pre, history=history)
+    res = solver.solve(6)
+    fwd = lambda x: orc.forward(x.reshape(mshape)).ravel()  # noqa: E731
+    adj = lambda y: orc.adjoint(y.reshape(d.shape)).ravel()  # noqa: E731
+    mo, ro, _ = O.fcg_solve((fwd, adj), d, 6, history, precond=lambda g, it: P[it] * g)
+    hist = [1.0] + [np.sqrt(x) / np.linalg.norm(d) for x in ro]
+    announce(f"  device {np.round(res.resid_hist, 5)} oracle {np.round(hist, 5)}")
+    assert res.fallbacks == 0 and res.iterations == 6
+    np.testing.assert_allclose(res.resid_hist, hist, rtol=2e-3)
+    assert rel(solver.return_model().numpy(), mo) < 2e-3
+    op.close()
+
+
+def test_flexible_cg_history_is_cg_without_preconditioner(dev):
+    """Without a preconditioner the history terms vanish: flexible CG with
+    four steps follows cgstep."""
+    w = small_workload(nt=60, n_side=32)
+    mbox, mu = setup(dev, w)
+    op = RtmZeroOp(dev, w, store="f32", model_box=mbox, mute=mu)
+    op.prepare()
+    mv = op.model_vector(np.random.default_rng(4).standard_normal(op.model_size))
+    dv = op.data_vector()
+    op.forward(False, mv, dv)
+    a = SimpleSolver(cgstep(), dv, op).solve(6)
+    b = SimpleSolver(cgstep(), dv, op, history=4).solve(6)
+    np.testing.assert_allclose(b.resid_hist, a.resid_hist, rtol=1e-4)
+    with pytest.raises(RuntimeError):
+        SimpleSolver(cgstep(), dv, op, history=99).solve(2)
+    op.close()
+
+
+@pytest.mark.parametrize("store,tol", [("f32", 1e-5), ("bf16", 1e-2)])
+def test_illumination_matches_oracle(dev, store, tol):
+    """sum over shots and snapshots of s^2, windowed to the model box and
+    muted, against the oracle's FP64 snapshots."""
+    announce(f"illumination store={store}")
+    w = small_workload(nt=60, n_side=32, store=store, jimg=2 if store == "bf16" else 1)
+    mbox, mu = setup(dev, w)
+    op = RtmZeroOp(dev, w, store=store, model_box=mbox, mute=mu)
+    op.prepare()
+    orc = oracle_for(w, mbox, mu)
+    acc = np.zeros(int(np.prod(w.n)))
+    for S in orc.snaps:
+        for s in S:
+            acc += np.asarray(s, np.float64).ravel() ** 2
+    m1, m2, m3, o1, o2, o3 = mbox
+    n1, n2, n3 = w.n
+    ref = acc.reshape(n3, n2, n1)[o3:o3 + m3, o2:o2 + m2, o1:o1 + m1] * mu
+    got = op.illumination().numpy().reshape(m3, m2, m1)
+    assert rel(got, ref) < tol
+    op.close()

@@ -339,6 +339,51 @@ def test_cgstep_matches_reference_cgstep():
     np.testing.assert_allclose(res, res64, rtol=2e-4)
 
 
+def test_fcg_history1_is_cgstep():
+    r = rng(21)
+    A = r.standard_normal((50, 15))
+    d = r.standard_normal(50)
+    m1, res1 = O.cgstep_solve(A, d, 10)
+    m2, res2, _ = O.fcg_solve(A, d, 10, history=1)
+    np.testing.assert_allclose(res2, res1, rtol=1e-10)
+    np.testing.assert_allclose(m2, m1, rtol=1e-9, atol=1e-12)
+
+
+def test_fcg_fixed_preconditioner_keeps_cg_iterates():
+    """With a fixed SPD preconditioner the extra history coefficients vanish
+    (the steps are already A-conjugate): any history gives the PCG iterates."""
+    r = rng(22)
+    A = r.standard_normal((60, 20)) * np.linspace(0.2, 3.0, 20)
+    d = r.standard_normal(60)
+    w = r.uniform(0.5, 2.0, 20)
+    _, res1, _ = O.fcg_solve(A, d, 8, history=1, precond=lambda g, it: w * g)
+    _, res4, _ = O.fcg_solve(A, d, 8, history=4, precond=lambda g, it: w * g)
+    np.testing.assert_allclose(res4, res1, rtol=1e-8)
+
+
+def test_fcg_full_history_minimises_over_all_directions():
+    """With a preconditioner that changes every iteration and a history at
+    least as long as the run, iterate k minimises |A m - d| over the span of
+    every direction used so far (checked by lstsq on that span); truncated
+    to one step (cgstep) it does no better."""
+    r = rng(23)
+    A = r.standard_normal((70, 25))
+    d = r.standard_normal(70)
+    P = [r.uniform(0.1, 3.0, 25) for _ in range(12)]
+    pre = lambda g, it: P[it] * g  # noqa: E731
+    _, res, dirs = O.fcg_solve(A, d, 12, history=12, precond=pre)
+    for k in range(1, 13):
+        Z = np.stack(dirs[:k], axis=1)
+        c = np.linalg.lstsq(A @ Z, d, rcond=None)[0]
+        best = np.sum((A @ Z @ c - d) ** 2)
+        assert abs(res[k - 1] - best) <= 1e-9 * (d @ d), k
+    _, res1, _ = O.fcg_solve(A, d, 12, history=1, precond=pre)
+    assert res[-1] <= res1[-1] * (1 + 1e-9)
+    # and with n directions it reaches the least-squares solution
+    m, _, _ = O.fcg_solve(A, d, 25, history=25, precond=lambda g, it: P[it % 12] * g)
+    np.testing.assert_allclose(m, np.linalg.lstsq(A, d, rcond=None)[0], rtol=1e-6, atol=1e-9)
+
+
 # --- geometry ------------------------------------------------------------------
 def test_source_geometry_hand_example():
     locs, tr = O.source_geometry(10, 12, 5.0, 4.0, 6.0, np.array([1.0, 2.0]), 8)
