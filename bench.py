@@ -107,8 +107,9 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="override nt (testing only)")
     ap.add_argument("--ty", type=int, default=0, help="tile rows (0 = per-variant default)")
     ap.add_argument("--target-blocks", type=int, default=2048)
-    ap.add_argument("--x-per-lane", type=int, default=1, choices=(1, 2, 4),
-                    help="cells per lane of the forward-form step kernel")
+    ap.add_argument("--x-per-lane", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="cells per lane of the forward-form step kernel (0: automatic, "
+                         "4 at 512^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
@@ -157,7 +158,8 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (a.traffic_bytes or None),
-            "kernel": {1: "brtm::k_step (VL, fused sponge)",
+            "kernel": {0: "brtm::k_step_x4 (VL, 4 cells/lane, fused sponge)",
+                       1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
                        4: "brtm::k_step_x4 (VL, 4 cells/lane, fused sponge)"}[a.x_per_lane],
             "kernel_ms": round(k_ms, 4), "launches": kcount,

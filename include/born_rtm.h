@@ -170,10 +170,16 @@ int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 
 /* Cells per lane along x for VL-form steps: 1 (64-wide tiles, 4-byte
  * accesses), 2 (128-wide tiles, 8-byte accesses; n1 even) or 4 (256-wide
- * tiles, 16-byte accesses; n1 % 4 == 0).  A grid that does not allow the
- * width asked for uses the widest one it allows.  Results agree to FP32
- * rounding. */
+ * tiles, 16-byte accesses; n1 % 4 == 0); 0 (the default) picks the widest
+ * whose tiles keep at least 3/4 of the lanes busy.  A grid that does not
+ * allow the width asked for uses the widest one it allows.  Results agree
+ * to FP32 rounding. */
 int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
+
+/* One-cell VL steps staged through a ring of six LDS planes whose halos are
+ * loaded together with the z-queue (on != 0), or through the two-plane
+ * buffer that loads each halo one slice ahead (on = 0).  Same results. */
+int brtm_set_halo_ring(brtm_ctx *ctx, int on);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms

@@ -101,12 +101,23 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   // Tile rows / workgroup target per variant (measured on MI355X at 512^3):
   // plain steps prefer tall tiles, 16-bit snapshot variants short ones.
   int ty = c->ty_tile, tb = c->target_blocks;
-  // cells per lane: 4 or 2 where asked for and n1 allows, else 1
+  // cells per lane: 4 or 2 where asked for and n1 allows, else 1; automatic
+  // (0): the widest whose tiles keep at least 3/4 of the lanes busy
   int xl = 1;
-  if (form == FORM_VL && c->x_per_lane == 4 && c->n1 % 4 == 0) xl = 4;
-  else if (form == FORM_VL && c->x_per_lane >= 2 && c->n1 % 2 == 0) xl = 2;
+  if (form == FORM_VL) {
+    int want = c->x_per_lane;
+    if (want == 0) {
+      want = 1;
+      for (int k : {4, 2}) {
+        const int w = 64 * k, used = ((c->n1 + w - 1) / w) * w;
+        if (c->n1 % k == 0 && 4 * c->n1 >= 3 * used) { want = k; break; }
+      }
+    }
+    if (want == 4 && c->n1 % 4 == 0) xl = 4;
+    else if (want >= 2 && c->n1 % 2 == 0) xl = 2;
+  }
   if (ty == 0) {
-    if (xl == 4) { ty = 8; tb = 2048; }
+    if (xl == 4) { ty = 16; tb = 256; }
     else if (xl == 2) { ty = 16; tb = 2048; }
     else if (act == ACT_NONE) { ty = 32; tb = 1024; }
     else if (fmt == FMT_F32) { ty = 16; tb = 8192; }
@@ -126,7 +137,9 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  if (xl == 4)
+  if (xl == 1 && form == FORM_VL && c->halo_ring)
+    BRTM_HIP(launch_step_ring(act, fmt, P, ty, c->stream));
+  else if (xl == 4)
     BRTM_HIP(launch_step_x4(act, fmt, P, ty, c->stream));
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
@@ -792,10 +805,16 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   return BRTM_OK;
 }
 
+int brtm_set_halo_ring(brtm_ctx *c, int on) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  c->halo_ring = on != 0;
+  return BRTM_OK;
+}
+
 int brtm_set_x_per_lane(brtm_ctx *c, int cells) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
-  BRTM_REQUIRE(cells == 1 || cells == 2 || cells == 4, BRTM_EINVAL,
-               "cells per lane must be 1, 2 or 4");
+  BRTM_REQUIRE(cells == 0 || cells == 1 || cells == 2 || cells == 4, BRTM_EINVAL,
+               "cells per lane must be 0 (automatic), 1, 2 or 4");
   c->x_per_lane = cells;
   return BRTM_OK;
 }

@@ -39,6 +39,8 @@ struct StepParams {
 };
 
 hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// The VL step with a six-plane LDS ring (halos loaded with the z-queue).
+hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with two consecutive x per lane (128-wide tiles, n1 even).
 hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with four consecutive x per lane (256-wide tiles, n1 % 4 == 0).

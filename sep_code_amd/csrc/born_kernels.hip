@@ -264,6 +264,191 @@ k_step(const StepParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// The VL step with an LDS ring of six x-y planes.  Each tile loads the halo
+// of slice z + 5 in the same iteration as its own (and its neighbours') z-queue
+// load of that slice, so a halo line is fetched while the neighbouring tile's
+// copy of it is still in L2; the plane is used four iterations later.  Slot
+// (z + 4) % 6 is written at iteration z, slot z % 6 read: the slot written was
+// last read at iteration z - 2, before the barrier every wave has passed, so
+// one barrier per slice still suffices.
+// ---------------------------------------------------------------------------
+template <int TY, int RPT, int FMT, int ACT>
+__global__ void __launch_bounds__(64 * (TY / RPT))
+k_step_ring(const StepParams P) {
+  static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  constexpr int NWV = TY / RPT;
+  constexpr int HY = 8 / NWV;
+  static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
+  constexpr int LW = 72;
+  constexpr int NR = 6;
+  __shared__ float ring[NR][TY + 8][LW];
+  __shared__ float wsh[kWMax];
+
+  const int lane = threadIdx.x;
+  const int wy = threadIdx.y;
+  const int n1 = P.n1, n2 = P.n2, n3 = P.n3, nw = P.nw;
+  const int s3 = n1 * n2;
+  for (int e = wy * 64 + lane; e < nw; e += 64 * NWV) wsh[e] = P.wtab[e];
+
+  const int nb = P.ntx * P.nty * P.ntz;
+  const int bid = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = bid & 7, jj = bid >> 3;
+  const int t = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const int tx = t % P.ntx;
+  const int ty = (t / P.ntx) % P.nty;
+  const int tz = t / (P.ntx * P.nty);
+
+  const int x = tx * 64 + lane;
+  const int y0 = ty * TY;
+  const int yb = y0 + wy * RPT;
+  const int z0 = 4 + tz * P.zc;
+  const int z1 = min(z0 + P.zc, n3 - 4);
+  const bool xmem = x < n1;
+  const bool xin = (x >= 4) && (x < n1 - 4);
+  const int ex = edist(x, n1);
+
+  auto ld = [&](const float *a, int i, bool ok) -> float { return ok ? a[i] : 0.f; };
+
+  float q[RPT][9];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = yb + r;
+    const bool ok = xmem && y < n2;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q[r][k] = ld(P.p, (z0 - 4 + k) * s3 + y * n1 + x, ok);
+  }
+
+  int hyy[HY], hyrow[HY];
+#pragma unroll
+  for (int j = 0; j < HY; ++j) {
+    const int h = wy * HY + j;
+    hyy[j] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+    hyrow[j] = h < 4 ? h : TY + h;
+  }
+  const bool hxl = lane < 8;
+  const int hxx = lane < 4 ? tx * 64 - 4 + lane : tx * 64 + 60 + lane;
+  const int hxcol = lane < 4 ? lane : 64 + lane;
+  const bool hxok = hxl && hxx >= 0 && hxx < n1;
+
+  auto load_halo = [&](int z, float *ohy, float *ohx) {
+#pragma unroll
+    for (int j = 0; j < HY; ++j)
+      ohy[j] = ld(P.p, z * s3 + hyy[j] * n1 + x, xmem && hyy[j] >= 0 && hyy[j] < n2);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      ohx[r] = ld(P.p, z * s3 + (yb + r) * n1 + hxx, hxok && (yb + r) < n2);
+  };
+  // the plane of slice z: centre values c[r], halos hy / hx
+  auto put_plane = [&](int z, const float *c, const float *hy, const float *hx) {
+    float(*pl)[LW] = ring[z % NR];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) pl[4 + wy * RPT + r][lane + 4] = c[r];
+#pragma unroll
+    for (int j = 0; j < HY; ++j) pl[hyrow[j]][lane + 4] = hy[j];
+    if (hxl) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) pl[4 + wy * RPT + r][hxcol] = hx[r];
+    }
+  };
+  auto load_center = [&](int z, float *opo, float *oV, float *oS, float *oX) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const bool ok = xmem && y < n2;
+      const int i = z * s3 + y * n1 + x;
+      opo[r] = ld(P.po, i, ok);
+      oV[r] = ld(P.V, i, ok);
+      if constexpr (ACT != ACT_NONE) {
+        oS[r] = ok ? snap_at<FMT>(P.snap, P.sscale, i, z) : 0.f;
+        oX[r] = ok ? (ACT == ACT_IMG ? P.img[i] : P.m[i]) : 0.f;
+      }
+    }
+  };
+
+  // prologue: planes z0 .. z0+3 into the ring; the halo of z0+4 waits in hy/hx
+  float hy[HY], hx[RPT];
+  {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float th[HY], tx_[RPT], c[RPT];
+      load_halo(z0 + k, th, tx_);
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) c[r] = q[r][4 + k];
+      put_plane(z0 + k, c, th, tx_);
+    }
+    load_halo(z0 + 4, hy, hx);
+  }
+  float cpo[RPT], cV[RPT], cS[RPT], cX[RPT];
+  load_center(z0, cpo, cV, cS, cX);
+
+  const float c0 = P.c[0];
+  for (int z = z0; z < z1; ++z) {
+    {
+      float c[RPT];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) c[r] = q[r][8];
+      put_plane(z + 4, c, hy, hx);
+    }
+    const int zn = z + 1;
+    float nq[RPT], nhy[HY], nhx[RPT];
+    float npo[RPT], nV[RPT], nS[RPT], nX[RPT];
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int y = yb + r;
+        nq[r] = ld(P.p, (zn + 4) * s3 + y * n1 + x, xmem && y < n2);
+      }
+      load_halo(zn + 4, nhy, nhx);
+      load_center(zn, npo, nV, nS, nX);
+    }
+    __syncthreads();
+
+    float(*pl)[LW] = ring[z % NR];
+    const int ez = edist(z, n3);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const int row = 4 + wy * RPT + r;
+      const int col = lane + 4;
+      const float cc = q[r][4];
+      float acc = c0 * cc;
+#pragma unroll
+      for (int k = 1; k <= 4; ++k) {
+        acc = fmaf(P.c[1 + 3 * (k - 1)], pl[row][col - k] + pl[row][col + k], acc);
+        acc = fmaf(P.c[2 + 3 * (k - 1)], pl[row - k][col] + pl[row + k][col], acc);
+        acc = fmaf(P.c[3 + 3 * (k - 1)], q[r][4 - k] + q[r][4 + k], acc);
+      }
+      if (xin && y >= 4 && y < n2 - 4) {
+        const int e = min(min(ex, edist(y, n2)), ez);
+        const float w = e < nw ? wsh[e] : 1.f;
+        const int i = z * s3 + y * n1 + x;
+        float pn = w * fmaf(cV[r], acc, 2.f * cc) - (w * w) * cpo[r];
+        if constexpr (ACT == ACT_BORN) pn = fmaf(P.cimg * cS[r], cX[r], pn);
+        __builtin_nontemporal_store(pn, P.pn + i);
+        if constexpr (ACT == ACT_IMG)
+          __builtin_nontemporal_store(fmaf(P.cimg * cS[r], pn, cX[r]), P.img + i);
+      }
+    }
+
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[r][k] = q[r][k + 1];
+        q[r][8] = nq[r];
+        cpo[r] = npo[r];
+        cV[r] = nV[r];
+        cS[r] = nS[r];
+        cX[r] = nX[r];
+        hx[r] = nhx[r];
+      }
+#pragma unroll
+      for (int j = 0; j < HY; ++j) hy[j] = nhy[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The VL step with two consecutive x per lane (n1 even): a wave row covers
 // 128 floats with 8-byte loads and stores, and the x taps of the two cells
 // come from four 8-byte LDS reads instead of sixteen 4-byte ones.
@@ -1020,6 +1205,32 @@ hipError_t launch_x2_t(int act, int fmt, const StepParams &P, hipStream_t st) {
 }
 
 template <int TY, int RPT, int ACT>
+hipError_t launch_ring_a(int fmt, const StepParams &P, hipStream_t st) {
+  const dim3 block(64, TY / RPT);
+  const dim3 grid(P.ntx * P.nty * P.ntz);
+  switch (fmt) {
+    case FMT_F32: k_step_ring<TY, RPT, FMT_F32, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_BF16: k_step_ring<TY, RPT, FMT_BF16, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_F16: k_step_ring<TY, RPT, FMT_F16, ACT><<<grid, block, 0, st>>>(P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int TY, int RPT>
+hipError_t launch_ring_t(int act, int fmt, const StepParams &P, hipStream_t st) {
+  if (act == ACT_NONE) {
+    if (fmt != FMT_NONE) return hipErrorInvalidValue;
+    k_step_ring<TY, RPT, FMT_NONE, ACT_NONE><<<dim3(P.ntx * P.nty * P.ntz), dim3(64, TY / RPT),
+                                               0, st>>>(P);
+    return hipGetLastError();
+  }
+  if (act == ACT_BORN) return launch_ring_a<TY, RPT, ACT_BORN>(fmt, P, st);
+  if (act == ACT_IMG) return launch_ring_a<TY, RPT, ACT_IMG>(fmt, P, st);
+  return hipErrorInvalidValue;
+}
+
+template <int TY, int RPT, int ACT>
 hipError_t launch_x4_a(int fmt, const StepParams &P, hipStream_t st) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.nty * P.ntz);
@@ -1080,6 +1291,13 @@ hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream
   if (ty_tile == 32) return launch_x2_t<32, 4>(act, fmt, P, st);
   if (ty_tile == 8) return launch_x2_t<8, 2>(act, fmt, P, st);
   return launch_x2_t<16, 2>(act, fmt, P, st);
+}
+
+hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+  if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 9) return hipErrorInvalidValue;
+  if (ty_tile == 32) return launch_ring_t<32, 4>(act, fmt, P, st);
+  if (ty_tile == 8) return launch_ring_t<8, 2>(act, fmt, P, st);
+  return launch_ring_t<16, 4>(act, fmt, P, st);
 }
 
 hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {

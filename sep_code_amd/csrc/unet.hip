@@ -116,6 +116,8 @@ struct brtm_unet {
   DevBuf partials, lossv, xpatch, ypatch;
   int step = 0;
   int gm1 = 0, gm2 = 0, gm3 = 0, gov = 8;
+  const brtm_vec *split_w = nullptr;  // split weight W of the preconditioner (not owned)
+  brtm_vec *split_tmp = nullptr;      // W . g
   float *prm() const { return params.as<float>(); }
 };
 
@@ -354,6 +356,7 @@ int brtm_unet_create(brtm_ctx *ctx, int batch, int patch, unsigned long long see
 int brtm_unet_destroy(brtm_unet *u) {
   if (!u) return BRTM_OK;
   (void)hipStreamSynchronize(u->ctx->stream);
+  brtm_vec_destroy(u->split_tmp);
   delete u;
   return BRTM_OK;
 }
@@ -482,11 +485,31 @@ int brtm_unet_set_grid(brtm_unet *u, int m1, int m2, int m3, int overlap) {
   return BRTM_OK;
 }
 
+int brtm_unet_set_split_weight(brtm_unet *u, const brtm_vec *w) {
+  BRTM_REQUIRE(u, BRTM_EINVAL, "NULL network");
+  if (w && (!u->split_tmp || brtm_vec_size(u->split_tmp) != brtm_vec_size(w))) {
+    brtm_vec_destroy(u->split_tmp);
+    u->split_tmp = nullptr;
+    U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->split_tmp));
+  }
+  u->split_w = w;
+  return BRTM_OK;
+}
+
 int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
   brtm_unet *u = static_cast<brtm_unet *>(user);
   if (!u || u->gm1 == 0) {
     set_error("brtm_unet_precond: call brtm_unet_set_grid first");
     return BRTM_ESTATE;
+  }
+  if (u->split_w) {
+    // z = W . (-s net(-W g / s)), s = rms(W g)
+    BRTM_REQUIRE(brtm_vec_size(u->split_w) == brtm_vec_size(g), BRTM_EINVAL,
+                 "split weight does not match the model grid");
+    U_TRY(brtm_vec_copy(u->split_tmp, g));
+    U_TRY(brtm_vec_mult(u->split_tmp, u->split_w));
+    U_TRY(apply_signed(u, u->split_tmp, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f));
+    return brtm_vec_mult(z, u->split_w);
   }
   // The solver's gradient is g = L^T (L m - d) = -(RTM image of the data
   // residual); the network maps RTM images to LSRTM images, so it is given -g

@@ -184,6 +184,10 @@ class BornDevice:
     def set_tuning(self, ty_tile=0, target_blocks=2048):
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
+    def set_halo_ring(self, on):
+        """One-cell VL steps on the six-plane LDS ring (halos loaded with the z-queue)."""
+        call("brtm_set_halo_ring", self.h, int(bool(on)))
+
     def set_x_per_lane(self, cells):
         """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1
         even); 4: 256-wide tiles with 16-byte accesses (VL, n1 % 4 == 0)."""

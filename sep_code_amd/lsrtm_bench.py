@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--diag-eps", default="",
                     help="comma list: also run CG preconditioned by the illumination "
                          "diagonal (I + eps max I)^-1 for each eps")
+    ap.add_argument("--split-eps", type=float, default=0.0,
+                    help="> 0: the learned run is z = W P(W g), W = (I + eps max I)^-1/2 "
+                         "(the network trained with the same split)")
     a = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
 
@@ -71,10 +74,12 @@ def main():
         net = UNet(dev, batch=a.batch, patch=64, seed=1)
         net.load(a.weights)
         dims = lsrtm.model_box(w)[:3]
-        pre = net.precond(dims, overlap=8)
+        W = lsrtm.illumination_weight(op, a.split_eps, power=0.5) if a.split_eps > 0 else None
+        pre = net.precond(dims, overlap=8, split_weight=W)
         r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="fpcg",
                                noise=e, precond=pre, history=a.history)
-        out["fpcg"] = {"history": a.history, "iterations": r.iterations, "converged": r.converged,
+        out["fpcg"] = {"history": a.history, "split_eps": a.split_eps,
+                       "iterations": r.iterations, "converged": r.converged,
                        "seconds": r.seconds, "judged": r.rel_resid_recomputed,
                        "fallbacks": r.fallbacks, "hist": r.resid_hist[::5]}
         log(f"learned FPCG: {r.iterations} it, {r.seconds:.1f}s, "

@@ -10,7 +10,7 @@ kernels' counters), then --reps launches each of the stencil variants
 and prints their HIP-event mean durations and algorithmic bandwidths, for each
 cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 8-byte accesses; 4: 256-wide tiles with 16-byte accesses; names get an
-"_x2" / "_x4" suffix).
+"_x2" / "_x4" suffix) and halo-ring setting in --ring (suffix "_ring").
 """
 from __future__ import annotations
 
@@ -34,6 +34,7 @@ def main():
     ap.add_argument("--ty", type=int, default=0)
     ap.add_argument("--variants", default="all")
     ap.add_argument("--xl", default="1,2")
+    ap.add_argument("--ring", default="0", help="comma list of halo-ring settings (0, 1)")
     a = ap.parse_args()
     n = a.n
     dev = BornDevice(0)
@@ -44,14 +45,19 @@ def main():
     nbytes = a.copy_gib * 2 ** 30
     ms = dev.copy_bench(nbytes, a.reps)
     out["copy"] = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / ms / 1e6}
-    for xl in [int(v) for v in a.xl.split(",")]:
-        dev.set_x_per_lane(xl)
-        for name, form, act, fmt, B in VARIANTS:
-            if a.variants != "all" and name not in a.variants.split(","):
+    for ring in [int(v) for v in a.ring.split(",")]:
+        dev.set_halo_ring(ring)
+        for xl in [int(v) for v in a.xl.split(",")]:
+            if ring and xl != 1:
                 continue
-            ms = dev.time_step(form, act, fmt, a.reps)
-            key = name if xl == 1 else f"{name}_x{xl}"
-            out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6, "gpts": n ** 3 / ms / 1e6}
+            dev.set_x_per_lane(xl)
+            for name, form, act, fmt, B in VARIANTS:
+                if a.variants != "all" and name not in a.variants.split(","):
+                    continue
+                ms = dev.time_step(form, act, fmt, a.reps)
+                key = name + (f"_x{xl}" if xl != 1 else "") + ("_ring" if ring else "")
+                out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
+                            "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()
 

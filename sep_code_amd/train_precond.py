@@ -32,23 +32,35 @@ from .operator import DeviceVector, SimpleSolver, cgstep
 from .unet import UNet
 
 
-def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(0,)):
+def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(0,),
+               split_eps=0.0):
     """One LSRTM run on the seeded model; returns ([(input, target)], dims,
-    relative residual after cg_iters)."""
+    relative residual after cg_iters).
+
+    split_eps > 0: the run is CG preconditioned by the illumination diagonal
+    W^2, W = (I + eps max I)^-1/2, and the pairs are amplitude-balanced for
+    the split preconditioner z = W P(W g):
+      input  = W (-g_k),   target = (m_20 - m_k) / W."""
     w = configs.lsrtm(seed=seed, n_side=n_side, nt=nt, anomalies=int(seed % 4))
     op, _ = lsrtm.setup_operator(dev, w)
     d, _ = lsrtm.observed_data(op, w, noise_frac=noise, seed=seed + 99)
     nm = op.model_size
     kept = {}
     solver = None
+    W = lsrtm.illumination_weight(op, split_eps, power=0.5) if split_eps > 0 else None
+    W2 = None
+    if W is not None:
+        W2 = W.clone().mult(W)
 
     def record(g_h, z_h):
-        # identity preconditioner (z = g: the plain cgstep iterates) that
-        # records g_k and m_k on the way
+        # the run's own preconditioner (z = g, or z = W^2 g) recording g_k
+        # and m_k on the way
         it = len(kept.setdefault("_calls", []))
         kept["_calls"].append(it)
         g = DeviceVector.view(dev, g_h, nm)
-        DeviceVector.view(dev, z_h, nm).copy_from(g)
+        z = DeviceVector.view(dev, z_h, nm).copy_from(g)
+        if W2 is not None:
+            z.mult(W2)
         if it in states:
             kept[it] = (g.numpy(), solver.model.numpy())
         return 0
@@ -57,6 +69,11 @@ def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(
     res = solver.solve(cg_iters)
     m20 = solver.model.numpy()
     pairs = [(-kept[k][0], m20 - kept[k][1]) for k in states if k in kept]
+    if W is not None:
+        wh = W.numpy()
+        pairs = [(wh * x, y / wh) for x, y in pairs]
+        W.close()
+        W2.close()
     dims = lsrtm.model_box(w)[:3]
     # rebinding the closure cell breaks the solver <-> callback cycle, so
     # the device vectors are released now, in order, by reference counting
@@ -108,6 +125,8 @@ def main():
                     help="CG iterations whose gradient / remaining update form pairs")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--split-eps", type=float, default=0.0,
+                    help="> 0: pairs for the illumination-split preconditioner")
     a = ap.parse_args()
     states = tuple(int(v) for v in a.states.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
@@ -115,7 +134,8 @@ def main():
     pairs = []
     t0 = time.perf_counter()
     for i in range(a.pairs):
-        ps, dims, rr = make_pairs(dev, a.seed0 + i, n_side=a.n, nt=a.nt, states=states)
+        ps, dims, rr = make_pairs(dev, a.seed0 + i, n_side=a.n, nt=a.nt, states=states,
+                                  split_eps=a.split_eps)
         pairs.extend((x, y, dims) for x, y in ps)
         log(f"model {i + 1}/{a.pairs}: {len(ps)} pairs, residual after 20 CG {rr:.3f} "
             f"({time.perf_counter() - t0:.1f}s)")
@@ -126,6 +146,7 @@ def main():
     net, hist = train(dev, pairs, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
     net.save(a.out)
     print(json.dumps({"models": a.pairs, "pairs": len(pairs), "states": list(states),
+                      "split_eps": a.split_eps,
                       "gen_seconds": tgen, "steps": a.steps,
                       "loss_first100": float(np.mean(hist[:100])),
                       "loss_last100": float(np.mean(hist[-100:]))}), flush=True)

@@ -30,6 +30,7 @@ _SIGS = {
     "brtm_unet_apply": [_P, _P, _P, _I, _I, _I, _I],
     "brtm_unet_set_grid": [_P, _I, _I, _I, _I],
     "brtm_unet_precond": [_P, _P, _P],
+    "brtm_unet_set_split_weight": [_P, _P],
 }
 _registered = False
 
@@ -134,10 +135,14 @@ class UNet:
         call("brtm_unet_apply", self.h, g.h, z.h, int(dims[0]), int(dims[1]), int(dims[2]),
              int(overlap))
 
-    def precond(self, dims, overlap=8):
-        """(native function address, user handle) for SimpleSolver."""
+    def precond(self, dims, overlap=8, split_weight: DeviceVector | None = None):
+        """(native function address, user handle) for SimpleSolver.  With
+        split_weight W the preconditioner is z = W P(W g) (keep W alive)."""
         call("brtm_unet_set_grid", self.h, int(dims[0]), int(dims[1]), int(dims[2]),
              int(overlap))
+        self._split_w = split_weight
+        call("brtm_unet_set_split_weight", self.h,
+             split_weight.h if split_weight is not None else None)
         lib = _reg()
         addr = C.cast(lib.brtm_unet_precond, C.c_void_p).value
         return ("native", addr, self.h.value)

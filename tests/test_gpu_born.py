@@ -67,16 +67,25 @@ def test_step_parity(dev, n, form, ty):
     dev.set_tuning(0, 2048)
 
 
-@pytest.mark.parametrize("xl", [2, 4])
+VARIANTS = [(2, 0), (4, 0), (1, 1)]  # (cells per lane, halo ring)
+
+
+def set_variant(dev, v):
+    dev.set_x_per_lane(v[0])
+    dev.set_halo_ring(v[1])
+
+
+@pytest.mark.parametrize("v", VARIANTS, ids=["x2", "x4", "ring"])
 @pytest.mark.parametrize("n", [(70, 36, 40), (130, 20, 24), (64, 64, 20), (71, 30, 24),
                                (260, 20, 24), (36, 40, 30)])
 @pytest.mark.parametrize("ty", [8, 16, 32])
-def test_step_parity_xl(dev, n, ty, xl):
+def test_step_parity_variants(dev, n, ty, v):
     """Forward-form step with two or four cells per lane (128- or 256-wide
-    tiles, 8- or 16-byte accesses): oracle parity and agreement with the
-    one-cell kernel.  Grids whose n1 the width does not divide (70 and 130
-    for four cells, 71 for both) run the widest kernel they allow."""
-    announce(f"x{xl} step parity n={n} ty={ty}")
+    tiles, 8- or 16-byte accesses) or on the six-plane halo ring: oracle
+    parity and agreement with the default kernel.  Grids whose n1 the width
+    does not divide (70 and 130 for four cells, 71 for both) run the widest
+    kernel they allow."""
+    announce(f"step parity variant={v} n={n} ty={ty}")
     r = np.random.default_rng(12)
     d = (10.0, 11.0, 12.0)
     V = velocity(r, n, d)
@@ -87,12 +96,12 @@ def test_step_parity_xl(dev, n, ty, xl):
     P = O.Problem(n, d, V, 7.0, 0.03)
     p1, p0 = random_field(r, n), random_field(r, n)
     g1, g1o = dev.propagate(0, 60, p1, p0)
-    dev.set_x_per_lane(xl)
+    set_variant(dev, v)
     try:
         c2, _ = dev.propagate(0, 1, p1, p0)
         g2, g2o = dev.propagate(0, 60, p1, p0)
     finally:
-        dev.set_x_per_lane(1)
+        set_variant(dev, (0, 0))
         dev.set_tuning(0, 2048)
     o1, _ = P.propagate(0, 1, p1.ravel(), p0.ravel())
     o_cur, o_old = P.propagate(0, 60, p1.ravel(), p0.ravel())
@@ -165,20 +174,20 @@ def test_born_forward_parity(dev, store, direct):
     assert rel(D, Do) < 1e-5
 
 
-@pytest.mark.parametrize("xl", [2, 4])
+@pytest.mark.parametrize("v", VARIANTS, ids=["x2", "x4", "ring"])
 @pytest.mark.parametrize("store", ["f32", "bf16", "f16"])
-def test_actions_xl(dev, store, xl):
+def test_actions_variants(dev, store, v):
     """Born modelling (fused Born source) and the adjoint (fused imaging) on the
-    two- and four-cells-per-lane kernels agree with the one-cell kernel, and
-    with the oracle for FP32 snapshots."""
-    announce(f"x{xl} born/adjoint store={store}")
+    two- and four-cells-per-lane kernels and the halo ring agree with the
+    default kernel, and with the oracle for FP32 snapshots."""
+    announce(f"born/adjoint variant={v} store={store}")
     P, r = make_shot(dev, n=(52, 44, 40), jimg=2)
     m = random_field(r, P.n)
     snaps, _ = P.source_prop()
     Do = P.born_forward(m.ravel(), snaps)
     out = {}
-    for k in (1, xl):
-        dev.set_x_per_lane(k)
+    for k in ((1, 0), v):
+        set_variant(dev, k)
         try:
             dev.source_prop(1, P.nt, 2, store=store)
             dev.set_model(m)
@@ -189,12 +198,13 @@ def test_actions_xl(dev, store, xl):
             dev.rtm_adjoint(1, P.nt, 2)
             out[k] = (D, dev.get_image())
         finally:
-            dev.set_x_per_lane(1)
-    assert rel(out[xl][0], out[1][0]) < 1e-6
-    assert rel(out[xl][1], out[1][1]) < 1e-6
+            set_variant(dev, (0, 0))
+    base = out[(1, 0)]
+    assert rel(out[v][0], base[0]) < 1e-6
+    assert rel(out[v][1], base[1]) < 1e-6
     if store == "f32":
-        assert rel(out[xl][0], Do) < 1e-5
-        assert rel(out[xl][1], P.rtm_adjoint(Do, snaps)) < 1e-4
+        assert rel(out[v][0], Do) < 1e-5
+        assert rel(out[v][1], P.rtm_adjoint(Do, snaps)) < 1e-4
 
 
 @pytest.mark.parametrize("store,bits", [("bf16", 8), ("f16", 11)])

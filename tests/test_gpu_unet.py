@@ -130,3 +130,26 @@ def test_apply_covers_volume_and_is_scale_equivariant(net):
     got, want = z[48:70, 48:90, 48:96], y[8:30, 8:50, 8:56]
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-2
     np.testing.assert_allclose(got, want, rtol=2e-2, atol=2e-2 * np.abs(want).max())
+
+
+def test_split_preconditioner_with_identity_network_is_the_diagonal(net):
+    """With a zero head the residual network is the identity, so the
+    preconditioner callback returns W^2 g with a split weight W, and g
+    without one (the sign and RMS scalings cancel)."""
+    from sep_code_amd.operator import PRECOND_FN, DeviceVector
+    net.set_params(_theta(net, head=0.0))
+    dims = (100, 90, 70)
+    r = np.random.default_rng(8)
+    g = r.standard_normal(dims[::-1]).astype(np.float32).ravel()
+    W = r.uniform(0.1, 1.0, g.size).astype(np.float32)
+    gv = DeviceVector.from_array(net.dev, g)
+    wv = DeviceVector.from_array(net.dev, W)
+    zv = DeviceVector(net.dev, g.size)
+    _, addr, user = net.precond(dims, overlap=8, split_weight=wv)
+    fn = PRECOND_FN(addr)
+    assert fn(user, gv.h, zv.h) == 0
+    want = W.astype(np.float64) ** 2 * g
+    np.testing.assert_allclose(zv.numpy(), want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    _, addr, user = net.precond(dims, overlap=8)
+    assert fn(user, gv.h, zv.h) == 0
+    np.testing.assert_allclose(zv.numpy(), g, rtol=1e-5, atol=1e-6 * np.abs(g).max())
