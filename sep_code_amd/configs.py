@@ -155,8 +155,12 @@ def rtm128(seed: int = 2024, nt: int = 1000) -> Workload:
                     [shot], ntr, store="f32")
 
 
-def rtm512(seed: int = 4096, nt: int = 4000) -> Workload:
-    n, d, dt, sponge = (512, 512, 512), (12.5, 12.5, 12.5), 0.001, 32
+def rtm512(seed: int = 4096, nt: int = 4000, n_side: int = 512, rec_step: int = 1,
+           store: str = "ckpt", jimg: int = 1) -> Workload:
+    """The 512^3 production shot; n_side < 512 gives the same physics on a
+    smaller grid (sponge, anomaly sizes and depths scaled with the grid)."""
+    f = n_side / 512.0
+    n, d, dt, sponge = (n_side,) * 3, (12.5, 12.5, 12.5), 0.001, max(8, round(32 * f))
     nbound = sponge + 4
     rng = SplitMix64(seed)
     vz, depths = _layered(rng, n, nbound, 5, 8, 1500.0, 4500.0)
@@ -166,8 +170,8 @@ def rtm512(seed: int = 4096, nt: int = 4000) -> Workload:
     x = np.arange(n[0], dtype=np.float32)[None, None, :]
     for _ in range(3):
         cx, cy = rng.uniform(nbound, n[0] - nbound), rng.uniform(nbound, n[1] - nbound)
-        cz = rng.uniform(nbound + 40, n[2] - nbound)
-        sig = rng.uniform(20.0, 60.0)
+        cz = rng.uniform(nbound + 40 * f, n[2] - nbound)
+        sig = rng.uniform(20.0, 60.0) * f
         amp = rng.uniform(-0.15, 0.15)
         g = np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / np.float32(2 * sig * sig))
         vel *= (1 + np.float32(amp) * g)
@@ -175,9 +179,10 @@ def rtm512(seed: int = 4096, nt: int = 4000) -> Workload:
     ntr = nt + 8
     w = G.wavelet_derivative(G.ricker(15.0, dt, nt), dt)
     c = n[0] / 2 - nbound
-    shot = _point_source_shot(n, nbound, w, ntr, c, c, 2.0)
-    return Workload("rtm512", n, d, dt, nt, 1, 1, float(sponge), 0.001, nbound, vel, refl,
-                    [shot], ntr, store="ckpt", ckpt=50)
+    shot = _point_source_shot(n, nbound, w, ntr, c, c, 2.0, rec_step=rec_step)
+    name = "rtm512" if n_side == 512 else f"rtm{n_side}p"
+    return Workload(name, n, d, dt, nt, 1, jimg, float(sponge), 0.001, nbound, vel, refl,
+                    [shot], ntr, store=store, ckpt=50)
 
 
 def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_step=2,
