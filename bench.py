@@ -10,11 +10,13 @@ before the timed region.
 value = RTM Gpts/s = 2 * n1*n2*n3 * nt / time per shot (the two wavefields
 an RTM must advance; the checkpoint recompute is overhead, not credited).
 
-roofline: the dominant kernel is the fused VL stencil step (k_step, no
-imaging; it runs for the source propagation and the recompute).  Its launches
-inside the timed region are bracketed by HIP events on the library stream;
-achieved = 16 B per cell (read p_n, p_{n-1}, V; write p_{n+1}) * n1*n2*n3 /
-mean launch duration.
+roofline: the dominant kernel is the fused VL stencil step (k_step_x4 at
+512^3: four cells per lane, no imaging; it runs for the source propagation
+and the recompute).  Its launches inside the timed region are bracketed by
+HIP events on the library stream; achieved = 16 B per cell (read p_n,
+p_{n-1}, V; write p_{n+1}) * n1*n2*n3 / mean launch duration.  traffic is
+the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+(scaled by a 1 GiB copy's counters) for the default configuration.
 
 cpu_baseline: the FP32 OpenMP CPU restatement (oracle/liborc32.so) of the
 same stencil step at 512^3, timed on the host cores for ~10 s.
@@ -34,6 +36,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 16.0
+# HBM bytes per launch of the plain VL step (k_step_x4, 512^3) from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration (2.0 / 1.0):
+# profiles/stencil512_traffic.md.  Used for the default configuration only.
+TRAFFIC_X4_512 = (2.0 * 922600 + 508032) * 1024.0
 
 
 def log(msg):
@@ -155,9 +161,12 @@ def main():
     value = 2.0 * ncell * w.nt * a.steps / el / 1e9
     k_ms = ktot / max(kcount, 1)
     achieved = BYTES_PER_CELL * ncell / (k_ms * 1e-3) / 1e9
+    traffic = a.traffic_bytes or None
+    if traffic is None and w.n == (512, 512, 512) and a.x_per_lane in (0, 4) and a.ty == 0:
+        traffic = TRAFFIC_X4_512
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (a.traffic_bytes or None),
+            "traffic": traffic,
             "kernel": {0: "brtm::k_step_x4 (VL, 4 cells/lane, fused sponge)",
                        1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",

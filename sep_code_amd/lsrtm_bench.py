@@ -43,6 +43,8 @@ def main():
                     help="trained network ('' to skip the learned run)")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--quiet", action="store_true",
+                    help="no per-iteration progress lines on stderr")
     ap.add_argument("--history", type=int, default=4,
                     help="steps kept by the flexible CG of the learned run")
     ap.add_argument("--diag-eps", default="",
@@ -65,7 +67,7 @@ def main():
            "store": w.store, "jimg": w.jimg, "tol": a.tol, "noise_rel": floor}
     if not a.skip_plain:
         r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="cg",
-                               noise=e, verbose=False)
+                               noise=e, verbose=not a.quiet)
         out["cg"] = {"iterations": r.iterations, "converged": r.converged,
                      "seconds": r.seconds, "judged": r.rel_resid_recomputed,
                      "hist": r.resid_hist[::5]}
@@ -75,7 +77,8 @@ def main():
         wv = lsrtm.illumination_weight(op, eps)
         tw = time.perf_counter() - t0
         r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="diag",
-                               noise=e, precond=lsrtm.diagonal_precond(dev, wv))
+                               noise=e, precond=lsrtm.diagonal_precond(dev, wv),
+                               verbose=not a.quiet)
         out[f"diag_{eps:g}"] = {"iterations": r.iterations, "converged": r.converged,
                                 "seconds": r.seconds + tw, "judged": r.rel_resid_recomputed,
                                 "hist": r.resid_hist[::5]}
@@ -92,7 +95,7 @@ def main():
         tw = time.perf_counter() - t0
         pre = net.precond(dims, overlap=8, split_weight=W)
         r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="fpcg",
-                               noise=e, precond=pre, history=a.history)
+                               noise=e, precond=pre, history=a.history, verbose=not a.quiet)
         out["fpcg"] = {"history": a.history, "split_eps": a.split_eps,
                        "iterations": r.iterations, "converged": r.converged,
                        "seconds": r.seconds + tw, "illumination_seconds": tw,
