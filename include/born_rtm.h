@@ -176,9 +176,10 @@ int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
  * to FP32 rounding. */
 int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
 
-/* One-cell VL steps staged through a ring of six LDS planes whose halos are
- * loaded together with the z-queue (on != 0), or through the two-plane
- * buffer that loads each halo one slice ahead (on = 0).  Same results. */
+/* One- and four-cell VL steps staged through a ring of six LDS planes whose
+ * halos are loaded together with the z-queue (on != 0), or through the
+ * two-plane buffer that loads each halo one slice ahead (on = 0).  Same
+ * results. */
 int brtm_set_halo_ring(brtm_ctx *ctx, int on);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes

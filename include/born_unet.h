@@ -61,7 +61,9 @@ int brtm_unet_set_grid(brtm_unet *net, int m1, int m2, int m3, int overlap);
  * root of the source illumination; NULL clears it): brtm_unet_precond then
  * returns z = W P(W g), so that P works on amplitude-balanced gradients and
  * W P W is the illumination-diagonal preconditioner when P is the identity.
- * The vector is not copied: keep it alive while it is set. */
+ * A network output that is not a descent direction (z.g <= 0 or not finite)
+ * is replaced by W^2 g.  The vector is not copied: keep it alive while it
+ * is set. */
 int brtm_unet_set_split_weight(brtm_unet *net, const brtm_vec *w);
 /* brtm_precond_fn for brtm_cg_solve: user = brtm_unet*.  The solver's
  * gradient g = L^T(L m - d) is minus the RTM image of the data residual, so

@@ -140,7 +140,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   if (xl == 1 && form == FORM_VL && c->halo_ring)
     BRTM_HIP(launch_step_ring(act, fmt, P, ty, c->stream));
   else if (xl == 4)
-    BRTM_HIP(launch_step_x4(act, fmt, P, ty, c->stream));
+    BRTM_HIP(launch_step_x4(act, fmt, P, ty, c->halo_ring, c->stream));
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else

@@ -43,8 +43,10 @@ hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hi
 hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with two consecutive x per lane (128-wide tiles, n1 even).
 hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
-// The VL step with four consecutive x per lane (256-wide tiles, n1 % 4 == 0).
-hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// The VL step with four consecutive x per lane (256-wide tiles, n1 % 4 == 0),
+// on the two-plane buffer or (ring) the six-plane halo ring.
+hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring,
+                          hipStream_t st);
 // Tile / chunk geometry chosen for a grid (fills ntx..zc of P); xw is the
 // tile width in x (64, 128 or 256 for the 1-, 2- and 4-cell kernels).
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw = 64);

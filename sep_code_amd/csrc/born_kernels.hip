@@ -884,6 +884,217 @@ k_step_x4(const StepParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Four cells per lane on the six-plane LDS ring (the two ideas above
+// together): 16-byte accesses, and each halo loaded in the same iteration as
+// the neighbouring tiles' z-queue loads of that slice.  16-row tiles take
+// 149 KB of LDS, which the one workgroup per CU that the registers allow
+// can have.
+// ---------------------------------------------------------------------------
+template <int TY, int RPT, int FMT, int ACT>
+__global__ void __launch_bounds__(64 * (TY / RPT))
+k_step_x4r(const StepParams P) {
+  static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  constexpr int NWV = TY / RPT;
+  constexpr int HY = 8 / NWV;
+  static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
+  constexpr int LW = 264;
+  constexpr int NR = 6;
+  __shared__ __attribute__((aligned(16))) float ring[NR][TY + 8][LW];
+  __shared__ float wsh[kWMax];
+
+  const int lane = threadIdx.x;
+  const int wy = threadIdx.y;
+  const int n1 = P.n1, n2 = P.n2, n3 = P.n3, nw = P.nw;
+  const int s3 = n1 * n2;
+  for (int e = wy * 64 + lane; e < nw; e += 64 * NWV) wsh[e] = P.wtab[e];
+
+  const int nb = P.ntx * P.nty * P.ntz;
+  const int bid = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = bid & 7, jj = bid >> 3;
+  const int t = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const int tx = t % P.ntx;
+  const int ty = (t / P.ntx) % P.nty;
+  const int tz = t / (P.ntx * P.nty);
+
+  const int x = tx * 256 + 4 * lane;
+  const int y0 = ty * TY;
+  const int yb = y0 + wy * RPT;
+  const int z0 = 4 + tz * P.zc;
+  const int z1 = min(z0 + P.zc, n3 - 4);
+  const bool xmem = x < n1;
+  const bool xin = (x >= 4) && (x < n1 - 4);
+  int ex[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ex[j] = edist(x + j, n1);
+
+  auto ld4 = [&](const float *a, int i, bool ok) -> f32x4 {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (ok) v = *reinterpret_cast<const f32x4 *>(a + i);
+    return v;
+  };
+
+  f32x4 q[RPT][9];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = yb + r;
+    const bool ok = xmem && y < n2;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q[r][k] = ld4(P.p, (z0 - 4 + k) * s3 + y * n1 + x, ok);
+  }
+
+  int hyy[HY], hyrow[HY];
+#pragma unroll
+  for (int j = 0; j < HY; ++j) {
+    const int h = wy * HY + j;
+    hyy[j] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+    hyrow[j] = h < 4 ? h : TY + h;
+  }
+  const bool hxl = lane < 2;
+  const int hxx = lane == 0 ? tx * 256 - 4 : tx * 256 + 256;
+  const int hxcol = lane == 0 ? 0 : 260;
+  const bool hxok = hxl && hxx >= 0 && hxx < n1;
+
+  auto load_halo = [&](int z, f32x4 *ohy, f32x4 *ohx) {
+#pragma unroll
+    for (int j = 0; j < HY; ++j)
+      ohy[j] = ld4(P.p, z * s3 + hyy[j] * n1 + x, xmem && hyy[j] >= 0 && hyy[j] < n2);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      ohx[r] = ld4(P.p, z * s3 + (yb + r) * n1 + hxx, hxok && (yb + r) < n2);
+  };
+  auto put_plane = [&](int z, const f32x4 *c, const f32x4 *hy, const f32x4 *hx) {
+    float(*pl)[LW] = ring[z % NR];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      *reinterpret_cast<f32x4 *>(&pl[4 + wy * RPT + r][4 + 4 * lane]) = c[r];
+#pragma unroll
+    for (int j = 0; j < HY; ++j) *reinterpret_cast<f32x4 *>(&pl[hyrow[j]][4 + 4 * lane]) = hy[j];
+    if (hxl) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) *reinterpret_cast<f32x4 *>(&pl[4 + wy * RPT + r][hxcol]) = hx[r];
+    }
+  };
+  auto load_center = [&](int z, f32x4 *opo, f32x4 *oV, f32x4 *oS, f32x4 *oX) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const bool ok = xmem && y < n2;
+      const int i = z * s3 + y * n1 + x;
+      opo[r] = ld4(P.po, i, ok);
+      oV[r] = ld4(P.V, i, ok);
+      if constexpr (ACT != ACT_NONE) {
+        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+        oS[r] = ok ? snap4_at<FMT>(P.snap, P.sscale, i, z) : zero;
+        oX[r] = ld4(ACT == ACT_IMG ? P.img : P.m, i, ok);
+      }
+    }
+  };
+
+  // prologue: planes z0 .. z0+3 into the ring; the halo of z0+4 waits in hy/hx
+  f32x4 hy[HY], hx[RPT];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f32x4 th[HY], tx_[RPT], c[RPT];
+    load_halo(z0 + k, th, tx_);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) c[r] = q[r][4 + k];
+    put_plane(z0 + k, c, th, tx_);
+  }
+  load_halo(z0 + 4, hy, hx);
+  f32x4 cpo[RPT], cV[RPT], cS[RPT], cX[RPT];
+  load_center(z0, cpo, cV, cS, cX);
+
+  const float c0 = P.c[0];
+  for (int z = z0; z < z1; ++z) {
+    {
+      f32x4 c[RPT];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) c[r] = q[r][8];
+      put_plane(z + 4, c, hy, hx);
+    }
+    const int zn = z + 1;
+    f32x4 nq[RPT], nhy[HY], nhx[RPT];
+    f32x4 npo[RPT], nV[RPT], nS[RPT], nX[RPT];
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int y = yb + r;
+        nq[r] = ld4(P.p, (zn + 4) * s3 + y * n1 + x, xmem && y < n2);
+      }
+      load_halo(zn + 4, nhy, nhx);
+      load_center(zn, npo, nV, nS, nX);
+    }
+    __syncthreads();
+
+    float(*pl)[LW] = ring[z % NR];
+    const int ez = edist(z, n3);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int y = yb + r;
+      const int row = 4 + wy * RPT + r;
+      const int col = 4 + 4 * lane;
+      const f32x4 C = q[r][4];
+      const f32x4 L = *reinterpret_cast<const f32x4 *>(&pl[row][col - 4]);
+      const f32x4 R = *reinterpret_cast<const f32x4 *>(&pl[row][col + 4]);
+      const float a[12] = {L.x, L.y, L.z, L.w, C.x, C.y, C.z, C.w, R.x, R.y, R.z, R.w};
+      float acc[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = c0 * a[4 + j];
+#pragma unroll
+      for (int k = 1; k <= 4; ++k) {
+        const f32x4 up = *reinterpret_cast<const f32x4 *>(&pl[row - k][col]);
+        const f32x4 dn = *reinterpret_cast<const f32x4 *>(&pl[row + k][col]);
+        const float c1 = P.c[1 + 3 * (k - 1)], c2 = P.c[2 + 3 * (k - 1)], c3 = P.c[3 + 3 * (k - 1)];
+        const f32x4 ud = up + dn;
+        const f32x4 zz = q[r][4 - k] + q[r][4 + k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[j] = fmaf(c1, a[4 + j - k] + a[4 + j + k], acc[j]);
+          acc[j] = fmaf(c2, ud[j], acc[j]);
+          acc[j] = fmaf(c3, zz[j], acc[j]);
+        }
+      }
+      if (xin && y >= 4 && y < n2 - 4) {
+        const int eyz = min(edist(y, n2), ez);
+        const int i = z * s3 + y * n1 + x;
+        f32x4 pn;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = min(ex[j], eyz);
+          const float w = e < nw ? wsh[e] : 1.f;
+          float v = w * fmaf(cV[r][j], acc[j], 2.f * C[j]) - (w * w) * cpo[r][j];
+          if constexpr (ACT == ACT_BORN) v = fmaf(P.cimg * cS[r][j], cX[r][j], v);
+          pn[j] = v;
+        }
+        __builtin_nontemporal_store(pn, reinterpret_cast<f32x4 *>(P.pn + i));
+        if constexpr (ACT == ACT_IMG) {
+          f32x4 im;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) im[j] = fmaf(P.cimg * cS[r][j], pn[j], cX[r][j]);
+          __builtin_nontemporal_store(im, reinterpret_cast<f32x4 *>(P.img + i));
+        }
+      }
+    }
+
+    if (zn < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[r][k] = q[r][k + 1];
+        q[r][8] = nq[r];
+        cpo[r] = npo[r];
+        cV[r] = nV[r];
+        cS[r] = nS[r];
+        cX[r] = nX[r];
+        hx[r] = nhx[r];
+      }
+#pragma unroll
+      for (int j = 0; j < HY; ++j) hy[j] = nhy[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Injection / extraction (one thread per point).
 // ---------------------------------------------------------------------------
 __global__ void k_inject(const InjectParams P) {
@@ -1232,29 +1443,36 @@ hipError_t launch_ring_t(int act, int fmt, const StepParams &P, hipStream_t st) 
   return hipErrorInvalidValue;
 }
 
-template <int TY, int RPT, int ACT>
-hipError_t launch_x4_a(int fmt, const StepParams &P, hipStream_t st) {
+template <bool RING, int TY, int RPT, int FMT, int ACT>
+void launch_x4_kernel(const StepParams &P, hipStream_t st) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.nty * P.ntz);
+  if constexpr (RING)
+    k_step_x4r<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
+  else
+    k_step_x4<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
+}
+
+template <bool RING, int TY, int RPT, int ACT>
+hipError_t launch_x4_a(int fmt, const StepParams &P, hipStream_t st) {
   switch (fmt) {
-    case FMT_F32: k_step_x4<TY, RPT, FMT_F32, ACT><<<grid, block, 0, st>>>(P); break;
-    case FMT_BF16: k_step_x4<TY, RPT, FMT_BF16, ACT><<<grid, block, 0, st>>>(P); break;
-    case FMT_F16: k_step_x4<TY, RPT, FMT_F16, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_F32: launch_x4_kernel<RING, TY, RPT, FMT_F32, ACT>(P, st); break;
+    case FMT_BF16: launch_x4_kernel<RING, TY, RPT, FMT_BF16, ACT>(P, st); break;
+    case FMT_F16: launch_x4_kernel<RING, TY, RPT, FMT_F16, ACT>(P, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int TY, int RPT>
+template <bool RING, int TY, int RPT>
 hipError_t launch_x4_t(int act, int fmt, const StepParams &P, hipStream_t st) {
   if (act == ACT_NONE) {
     if (fmt != FMT_NONE) return hipErrorInvalidValue;
-    k_step_x4<TY, RPT, FMT_NONE, ACT_NONE><<<dim3(P.ntx * P.nty * P.ntz), dim3(64, TY / RPT), 0,
-                                             st>>>(P);
+    launch_x4_kernel<RING, TY, RPT, FMT_NONE, ACT_NONE>(P, st);
     return hipGetLastError();
   }
-  if (act == ACT_BORN) return launch_x4_a<TY, RPT, ACT_BORN>(fmt, P, st);
-  if (act == ACT_IMG) return launch_x4_a<TY, RPT, ACT_IMG>(fmt, P, st);
+  if (act == ACT_BORN) return launch_x4_a<RING, TY, RPT, ACT_BORN>(fmt, P, st);
+  if (act == ACT_IMG) return launch_x4_a<RING, TY, RPT, ACT_IMG>(fmt, P, st);
   return hipErrorInvalidValue;
 }
 
@@ -1302,12 +1520,17 @@ hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStre
   return launch_ring_t<16, 4>(act, fmt, P, st);
 }
 
-hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring,
+                          hipStream_t st) {
   if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 12 || (P.n1 & 3)) return hipErrorInvalidValue;
   // 8-row tiles with one row per thread, or 16-row tiles with two (ty_tile
   // 16 or 32); four cells per lane leave no registers for taller tiles.
-  if (ty_tile == 8) return launch_x4_t<8, 1>(act, fmt, P, st);
-  return launch_x4_t<16, 2>(act, fmt, P, st);
+  if (ring) {
+    if (ty_tile == 8) return launch_x4_t<true, 8, 1>(act, fmt, P, st);
+    return launch_x4_t<true, 16, 2>(act, fmt, P, st);
+  }
+  if (ty_tile == 8) return launch_x4_t<false, 8, 1>(act, fmt, P, st);
+  return launch_x4_t<false, 16, 2>(act, fmt, P, st);
 }
 
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw) {

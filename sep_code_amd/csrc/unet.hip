@@ -509,7 +509,17 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
     U_TRY(brtm_vec_copy(u->split_tmp, g));
     U_TRY(brtm_vec_mult(u->split_tmp, u->split_w));
     U_TRY(apply_signed(u, u->split_tmp, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f));
-    return brtm_vec_mult(z, u->split_w);
+    U_TRY(brtm_vec_mult(z, u->split_w));
+    // Not a descent direction (or not finite): fall back to the diagonal
+    // W^2 g, what the split gives for an identity network, rather than
+    // leaving the solver to fall back to the raw gradient.
+    double zg = 0;
+    U_TRY(brtm_vec_dot(z, g, &zg));
+    if (!(zg > 0) || !std::isfinite(zg)) {
+      U_TRY(brtm_vec_copy(z, u->split_tmp));  // W g
+      U_TRY(brtm_vec_mult(z, u->split_w));    // W^2 g
+    }
+    return BRTM_OK;
   }
   // The solver's gradient is g = L^T (L m - d) = -(RTM image of the data
   // residual); the network maps RTM images to LSRTM images, so it is given -g

@@ -13,7 +13,8 @@ meets inside the solver:
 for k in --states (k = 0 is the plain pair).  Each volume is normalised by
 its own RMS (the CG search fixes the step length, so only the direction of
 P(g) matters).  The network is trained with its own forward/backward kernels
-and Adam on random 64^3 patches, with random sign (the map is odd).
+and Adam on random 64^3 patches, with random sign (the map is odd) and,
+optionally (--scale-aug), a random pair amplitude.
 
     python -m sep_code_amd.train_precond --pairs 96 --out weights.npy
 """
@@ -87,7 +88,7 @@ def normalise(a):
     return (a / r).astype(np.float32) if r > 0 else a
 
 
-def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print):
+def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print, scale_aug=1.0):
     net = UNet(dev, batch=batch, patch=64, seed=seed + 1)
     rng = np.random.default_rng(seed)
     vols = []
@@ -102,7 +103,12 @@ def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print):
         for k in range(batch):
             xv, yv, dims = vols[rng.integers(len(vols))]
             o = [int(rng.integers(-8, max(-7, dims[a] - P + 9))) for a in range(3)]
+            # random sign (the map is odd) and a log-uniform amplitude in
+            # [1/scale_aug, scale_aug] on the pair (the map is linear), so
+            # patches louder or quieter than the volume RMS stay in range
             sg = 1.0 if rng.random() < 0.5 else -1.0
+            if scale_aug > 1.0:
+                sg *= float(np.exp(rng.uniform(-1.0, 1.0) * np.log(scale_aug)))
             net.extract(xv, dims, o, sg, "x", k)
             net.extract(yv, dims, o, sg, "t", k)
         loss = net.train_step_device(batch, cur_lr)
@@ -125,6 +131,8 @@ def main():
                     help="CG iterations whose gradient / remaining update form pairs")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--scale-aug", type=float, default=1.0,
+                    help="> 1: random pair amplitude, log-uniform in [1/s, s]")
     ap.add_argument("--split-eps", type=float, default=0.0,
                     help="> 0: pairs for the illumination-split preconditioner")
     a = ap.parse_args()
@@ -143,10 +151,11 @@ def main():
         np.savez(a.save_pairs, **{f"x{i}": p[0] for i, p in enumerate(pairs)},
                  **{f"y{i}": p[1] for i, p in enumerate(pairs)})
     tgen = time.perf_counter() - t0
-    net, hist = train(dev, pairs, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
+    net, hist = train(dev, pairs, steps=a.steps, batch=a.batch, lr=a.lr, log=log,
+                      scale_aug=a.scale_aug)
     net.save(a.out)
     print(json.dumps({"models": a.pairs, "pairs": len(pairs), "states": list(states),
-                      "split_eps": a.split_eps,
+                      "split_eps": a.split_eps, "scale_aug": a.scale_aug,
                       "gen_seconds": tgen, "steps": a.steps,
                       "loss_first100": float(np.mean(hist[:100])),
                       "loss_last100": float(np.mean(hist[-100:]))}), flush=True)

@@ -67,7 +67,8 @@ def test_step_parity(dev, n, form, ty):
     dev.set_tuning(0, 2048)
 
 
-VARIANTS = [(2, 0), (4, 0), (1, 1)]  # (cells per lane, halo ring)
+VARIANTS = [(2, 0), (4, 0), (1, 1), (4, 1)]  # (cells per lane, halo ring)
+VARIANT_IDS = ["x2", "x4", "ring", "x4ring"]
 
 
 def set_variant(dev, v):
@@ -75,7 +76,7 @@ def set_variant(dev, v):
     dev.set_halo_ring(v[1])
 
 
-@pytest.mark.parametrize("v", VARIANTS, ids=["x2", "x4", "ring"])
+@pytest.mark.parametrize("v", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("n", [(70, 36, 40), (130, 20, 24), (64, 64, 20), (71, 30, 24),
                                (260, 20, 24), (36, 40, 30)])
 @pytest.mark.parametrize("ty", [8, 16, 32])
@@ -174,7 +175,7 @@ def test_born_forward_parity(dev, store, direct):
     assert rel(D, Do) < 1e-5
 
 
-@pytest.mark.parametrize("v", VARIANTS, ids=["x2", "x4", "ring"])
+@pytest.mark.parametrize("v", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("store", ["f32", "bf16", "f16"])
 def test_actions_variants(dev, store, v):
     """Born modelling (fused Born source) and the adjoint (fused imaging) on the
