@@ -62,9 +62,25 @@ int brtm_unet_set_grid(brtm_unet *net, int m1, int m2, int m3, int overlap);
  * returns z = W P(W g), so that P works on amplitude-balanced gradients and
  * W P W is the illumination-diagonal preconditioner when P is the identity.
  * A network output that is not a descent direction (z.g <= 0 or not finite)
- * is replaced by W^2 g.  The vector is not copied: keep it alive while it
- * is set. */
+ * is replaced by W^2 g, and so is one that stopped following its input (see
+ * brtm_unet_set_guard).  The vector is not copied: keep it alive while it is
+ * set.  Setting it (or the grid) starts a new solve for the guard. */
 int brtm_unet_set_split_weight(brtm_unet *net, const brtm_vec *w);
+/* Guard of the split preconditioner: the diagonal direction W^2 g replaces
+ * the network's z when cos(z, g) < min_cos (z brings nothing the gradient
+ * does not) or when cos(z, z_prev) > repeat_cos, z_prev being the network's
+ * previous output (the network has settled on a fixed direction, which
+ * stalls flexible CG).  Defaults 0.05 and 0.8.  brtm_unet_guard_count
+ * returns how many calls took W^2 g since the split weight was set. */
+int brtm_unet_set_guard(brtm_unet *net, double repeat_cos, double min_cos);
+int brtm_unet_guard_count(const brtm_unet *net);
+/* The preconditioner uses the odd part of the network,
+ * P(x) = (net(x) - net(-x)) / 2, on = 1 (the default for batch >= 2): the
+ * ideal preconditioner is linear, hence odd, and the even part -- a
+ * response of the biases that does not follow the input -- otherwise turns
+ * into a fixed direction that stalls the solver after a few iterations.
+ * Costs a second network evaluation per patch. */
+int brtm_unet_set_odd(brtm_unet *net, int on);
 /* brtm_precond_fn for brtm_cg_solve: user = brtm_unet*.  The solver's
  * gradient g = L^T(L m - d) is minus the RTM image of the data residual, so
  * the network is applied in its training orientation:

@@ -87,6 +87,23 @@ __global__ void k_insert(const float *__restrict__ src, int P, int ov, float *vo
   }
 }
 
+// As k_insert, of the odd part (net(x) - net(-x)) / 2 of two outputs.
+__global__ void k_insert_odd(const float *__restrict__ sp, const float *__restrict__ sm, int P,
+                             int ov, float *vol, int m1, int m2, int m3, int o1, int o2, int o3,
+                             float scale) {
+  const int S = P - 2 * ov;
+  const long n = (long)S * S * S;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int a = (int)(i % S) + ov, b = (int)((i / S) % S) + ov, c = (int)(i / ((long)S * S)) + ov;
+    const int x = o1 + a, y = o2 + b, z = o3 + c;
+    if (x >= 0 && x < m1 && y >= 0 && y < m2 && z >= 0 && z < m3) {
+      const long j = ((long)c * P + b) * P + a;
+      vol[((long)z * m2 + y) * m1 + x] = 0.5f * scale * (sp[j] - sm[j]);
+    }
+  }
+}
+
 struct Rng {
   unsigned long long s;
   unsigned long long next() {
@@ -118,6 +135,11 @@ struct brtm_unet {
   int gm1 = 0, gm2 = 0, gm3 = 0, gov = 8;
   const brtm_vec *split_w = nullptr;  // split weight W of the preconditioner (not owned)
   brtm_vec *split_tmp = nullptr;      // W . g
+  brtm_vec *znet_prev = nullptr;      // the network's previous output (guard)
+  bool have_prev = false;
+  double guard_repeat = 0.8, guard_min = 0.05;
+  int guard_uses = 0;                 // iterations that took W^2 g since the weight was set
+  bool odd = true;                    // preconditioner uses the odd part of the network
   float *prm() const { return params.as<float>(); }
 };
 
@@ -269,6 +291,7 @@ int brtm_unet_create(brtm_ctx *ctx, int batch, int patch, unsigned long long see
   brtm_unet *u = new brtm_unet();
   u->ctx = ctx;
   u->B = batch;
+  u->odd = batch >= 2;
   u->P = patch;
   u->P3 = (long)patch * patch * patch;
   long off = 0;
@@ -357,6 +380,7 @@ int brtm_unet_destroy(brtm_unet *u) {
   if (!u) return BRTM_OK;
   (void)hipStreamSynchronize(u->ctx->stream);
   brtm_vec_destroy(u->split_tmp);
+  brtm_vec_destroy(u->znet_prev);
   delete u;
   return BRTM_OK;
 }
@@ -424,9 +448,12 @@ int brtm_unet_extract(brtm_unet *u, const float *vol, int m1, int m2, int m3, in
 }
 
 // z = sign . rms(g) . net(sign . g / rms(g)), patch by patch.
+// odd: z = sign . rms . (net(x) - net(-x)) / 2 with x = sign . g / rms, the
+// odd part of the network (two evaluations per patch).
 static int apply_signed(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3, int ov,
-                 float sign) {
+                 float sign, bool odd = false) {
   BRTM_REQUIRE(u && g && z, BRTM_EINVAL, "NULL argument");
+  BRTM_REQUIRE(!odd || u->B >= 2, BRTM_EINVAL, "the odd part needs a batch of at least 2");
   BRTM_REQUIRE((long)m1 * m2 * m3 == brtm_vec_size(g) && brtm_vec_size(z) == brtm_vec_size(g),
                BRTM_EINVAL, "vector size does not match the model grid");
   BRTM_REQUIRE(ov >= 0 && 2 * ov < u->P, BRTM_EINVAL, "overlap too large for the patch");
@@ -450,20 +477,30 @@ static int apply_signed(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, in
   const float *gp = static_cast<const float *>(brtm_vec_data(const_cast<brtm_vec *>(g)));
   float *zp = static_cast<float *>(brtm_vec_data(z));
   hipStream_t st = u->ctx->stream;
-  for (int p0 = 0; p0 < np; p0 += u->B) {
-    const int nb = std::min(u->B, np - p0);
+  const int per = odd ? u->B / 2 : u->B;  // patches per forward pass
+  for (int p0 = 0; p0 < np; p0 += per) {
+    const int nb = std::min(per, np - p0);
     for (int k = 0; k < nb; ++k) {
       const int *o = &orig[3 * (p0 + k)];
       k_extract<<<1024, 256, 0, st>>>(gp, m1, m2, m3, o[0], o[1], o[2],
                                       sign * (float)(1.0 / rms),
                                       u->xpatch.as<float>() + k * u->P3, u->P);
+      if (odd)
+        k_extract<<<1024, 256, 0, st>>>(gp, m1, m2, m3, o[0], o[1], o[2],
+                                        -sign * (float)(1.0 / rms),
+                                        u->xpatch.as<float>() + (nb + k) * u->P3, u->P);
     }
     BRTM_HIP(hipGetLastError());
-    U_TRY(forward(u, u->xpatch.as<float>(), nb));
+    U_TRY(forward(u, u->xpatch.as<float>(), odd ? 2 * nb : nb));
     for (int k = 0; k < nb; ++k) {
       const int *o = &orig[3 * (p0 + k)];
-      k_insert<<<1024, 256, 0, st>>>(u->out.as<float>() + k * u->P3, u->P, ov, zp, m1, m2, m3,
-                                     o[0], o[1], o[2], sign * (float)rms);
+      const float *out = u->out.as<float>();
+      if (odd)
+        k_insert_odd<<<1024, 256, 0, st>>>(out + k * u->P3, out + (nb + k) * u->P3, u->P, ov, zp,
+                                           m1, m2, m3, o[0], o[1], o[2], sign * (float)rms);
+      else
+        k_insert<<<1024, 256, 0, st>>>(out + k * u->P3, u->P, ov, zp, m1, m2, m3, o[0], o[1],
+                                       o[2], sign * (float)rms);
     }
     BRTM_HIP(hipGetLastError());
   }
@@ -482,6 +519,7 @@ int brtm_unet_set_grid(brtm_unet *u, int m1, int m2, int m3, int overlap) {
   u->gm2 = m2;
   u->gm3 = m3;
   u->gov = overlap;
+  u->have_prev = false;
   return BRTM_OK;
 }
 
@@ -489,12 +527,26 @@ int brtm_unet_set_split_weight(brtm_unet *u, const brtm_vec *w) {
   BRTM_REQUIRE(u, BRTM_EINVAL, "NULL network");
   if (w && (!u->split_tmp || brtm_vec_size(u->split_tmp) != brtm_vec_size(w))) {
     brtm_vec_destroy(u->split_tmp);
-    u->split_tmp = nullptr;
+    brtm_vec_destroy(u->znet_prev);
+    u->split_tmp = u->znet_prev = nullptr;
     U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->split_tmp));
+    U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->znet_prev));
   }
   u->split_w = w;
+  u->have_prev = false;
+  u->guard_uses = 0;
   return BRTM_OK;
 }
+
+int brtm_unet_set_guard(brtm_unet *u, double repeat_cos, double min_cos) {
+  BRTM_REQUIRE(u && repeat_cos > 0 && repeat_cos <= 1 && min_cos >= 0 && min_cos < 1, BRTM_EINVAL,
+               "guard thresholds out of range");
+  u->guard_repeat = repeat_cos;
+  u->guard_min = min_cos;
+  return BRTM_OK;
+}
+
+int brtm_unet_guard_count(const brtm_unet *u) { return u ? u->guard_uses : 0; }
 
 int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
   brtm_unet *u = static_cast<brtm_unet *>(user);
@@ -508,14 +560,32 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
                  "split weight does not match the model grid");
     U_TRY(brtm_vec_copy(u->split_tmp, g));
     U_TRY(brtm_vec_mult(u->split_tmp, u->split_w));
-    U_TRY(apply_signed(u, u->split_tmp, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f));
+    U_TRY(apply_signed(u, u->split_tmp, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f, u->odd));
     U_TRY(brtm_vec_mult(z, u->split_w));
-    // Not a descent direction (or not finite): fall back to the diagonal
-    // W^2 g, what the split gives for an identity network, rather than
-    // leaving the solver to fall back to the raw gradient.
-    double zg = 0;
+    // Guard: use the diagonal W^2 g (what the split gives for an identity
+    // network) instead of the network's direction when that direction
+    //  * is not a descent direction or not finite;
+    //  * is nearly orthogonal to the gradient (cos < guard_min); or
+    //  * nearly repeats the network's previous output (cos > guard_repeat).
+    // The last two mark a network that has stopped following its input:
+    // on gradients unlike its training data its output settles on one fixed
+    // direction, which the search over previous steps has already used, and
+    // the solver stalls.
+    double zg = 0, zz = 0, gg = 0, zp = 0, pp = 0;
     U_TRY(brtm_vec_dot(z, g, &zg));
-    if (!(zg > 0) || !std::isfinite(zg)) {
+    U_TRY(brtm_vec_dot(z, z, &zz));
+    U_TRY(brtm_vec_dot(g, g, &gg));
+    bool use_diag = !(zg > 0) || !std::isfinite(zg) || !std::isfinite(zz) ||
+                    zg < u->guard_min * std::sqrt(zz * gg);
+    if (u->have_prev) {
+      U_TRY(brtm_vec_dot(z, u->znet_prev, &zp));
+      U_TRY(brtm_vec_dot(u->znet_prev, u->znet_prev, &pp));
+      if (pp > 0 && zz > 0 && zp > u->guard_repeat * std::sqrt(zz * pp)) use_diag = true;
+    }
+    U_TRY(brtm_vec_copy(u->znet_prev, z));
+    u->have_prev = true;
+    if (use_diag) {
+      ++u->guard_uses;
       U_TRY(brtm_vec_copy(z, u->split_tmp));  // W g
       U_TRY(brtm_vec_mult(z, u->split_w));    // W^2 g
     }
@@ -524,7 +594,14 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
   // The solver's gradient is g = L^T (L m - d) = -(RTM image of the data
   // residual); the network maps RTM images to LSRTM images, so it is given -g
   // and its output is negated back: z = -rms . net(-g / rms).
-  return apply_signed(u, g, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f);
+  return apply_signed(u, g, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f, u->odd);
+}
+
+int brtm_unet_set_odd(brtm_unet *u, int on) {
+  BRTM_REQUIRE(u, BRTM_EINVAL, "NULL network");
+  BRTM_REQUIRE(!on || u->B >= 2, BRTM_EINVAL, "the odd part needs a batch of at least 2");
+  u->odd = on != 0;
+  return BRTM_OK;
 }
 
 }  // extern "C"

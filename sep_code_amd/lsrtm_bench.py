@@ -43,6 +43,10 @@ def main():
                     help="trained network ('' to skip the learned run)")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--even", action="store_true",
+                    help="use the whole network, not its odd part")
+    ap.add_argument("--guard", default="0.8,0.05",
+                    help="repeat_cos,min_cos of the split preconditioner's guard")
     ap.add_argument("--quiet", action="store_true",
                     help="no per-iteration progress lines on stderr")
     ap.add_argument("--history", type=int, default=4,
@@ -89,6 +93,9 @@ def main():
         from .unet import UNet
         net = UNet(dev, batch=a.batch, patch=64, seed=1)
         net.load(a.weights)
+        net.set_odd(not a.even)
+        rep, mn = (float(v) for v in a.guard.split(","))
+        net.set_guard(rep, mn)
         dims = lsrtm.model_box(w)[:3]
         t0 = time.perf_counter()
         W = lsrtm.illumination_weight(op, a.split_eps, power=0.5) if a.split_eps > 0 else None
@@ -100,7 +107,8 @@ def main():
                        "iterations": r.iterations, "converged": r.converged,
                        "seconds": r.seconds + tw, "illumination_seconds": tw,
                        "judged": r.rel_resid_recomputed,
-                       "fallbacks": r.fallbacks, "hist": r.resid_hist[::5]}
+                       "fallbacks": r.fallbacks, "guard_diagonal_steps": net.guard_count(),
+                       "odd": not a.even, "guard": a.guard, "hist": r.resid_hist[::5]}
         log(f"learned FPCG: {r.iterations} it, {r.seconds + tw:.1f}s, "
             f"judged {r.rel_resid_recomputed:.4e}, fallbacks {r.fallbacks}")
         if "cg" in out and out["cg"]["converged"] and r.converged:

@@ -31,6 +31,9 @@ _SIGS = {
     "brtm_unet_set_grid": [_P, _I, _I, _I, _I],
     "brtm_unet_precond": [_P, _P, _P],
     "brtm_unet_set_split_weight": [_P, _P],
+    "brtm_unet_set_odd": [_P, _I],
+    "brtm_unet_set_guard": [_P, _D, _D],
+    "brtm_unet_guard_count": [_P],
 }
 _registered = False
 
@@ -134,6 +137,17 @@ class UNet:
     def apply(self, g: DeviceVector, z: DeviceVector, dims, overlap=8):
         call("brtm_unet_apply", self.h, g.h, z.h, int(dims[0]), int(dims[1]), int(dims[2]),
              int(overlap))
+
+    def set_guard(self, repeat_cos=0.8, min_cos=0.05):
+        """Thresholds of the split preconditioner's fall-back to W^2 g."""
+        call("brtm_unet_set_guard", self.h, float(repeat_cos), float(min_cos))
+
+    def guard_count(self) -> int:
+        return int(_reg().brtm_unet_guard_count(self.h))
+
+    def set_odd(self, on: bool):
+        """Use the odd part (net(x) - net(-x)) / 2 as the preconditioner (default)."""
+        call("brtm_unet_set_odd", self.h, int(bool(on)))
 
     def precond(self, dims, overlap=8, split_weight: DeviceVector | None = None):
         """(native function address, user handle) for SimpleSolver.  With

@@ -85,7 +85,8 @@ def stage_gpu(a):
     dev.set_sinc_table(1, sinc)
     dev.set_source(s.src_locs, s.src_traces)
     dev.set_receivers(s.rec_locs, D, ntr=w.ntr, nx=s.nrx, ny=s.nry)
-    out = {"grid": list(w.n), "nt": w.nt, "jimg": JIMG, "errors": {}}
+    out = {"grid": list(w.n), "nt": w.nt, "jimg": JIMG, "errors": {}, "vs_f32_store": {}}
+    f32img = None
     for store in ("f32", "bf16", "f16", "ckpt"):
         dev.source_prop(1, w.nt, JIMG, store=store, ckpt=50)
         dev.zero_image()
@@ -93,7 +94,12 @@ def stage_gpu(a):
         img = dev.get_image().astype(np.float64).ravel()
         err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
         out["errors"][store] = err
-        print(f"{store}: relative L2 image error {err:.3e}", file=sys.stderr, flush=True)
+        # the store's own error: against the same FP32 arithmetic with FP32 snapshots
+        if f32img is None:
+            f32img = img
+        out["vs_f32_store"][store] = float(np.linalg.norm(img - f32img) / np.linalg.norm(f32img))
+        print(f"{store}: relative L2 image error {err:.3e} (vs FP32 store "
+              f"{out['vs_f32_store'][store]:.3e})", file=sys.stderr, flush=True)
         dev.release_store(0)
     print(json.dumps(out), flush=True)
     dev.close()

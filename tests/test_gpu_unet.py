@@ -153,3 +153,38 @@ def test_split_preconditioner_with_identity_network_is_the_diagonal(net):
     _, addr, user = net.precond(dims, overlap=8)
     assert fn(user, gv.h, zv.h) == 0
     np.testing.assert_allclose(zv.numpy(), g, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+
+
+def test_split_guard_replaces_a_fixed_direction(net):
+    """A network whose output ignores its input (whole network, head weights
+    zero, a large head bias: net(x) = x + b) gives nearly the same direction
+    on every call; the guard then returns the diagonal direction W^2 g."""
+    from sep_code_amd.operator import PRECOND_FN, DeviceVector
+    from oracle import unet_oracle as U
+    theta = _theta(net, head=0.0)
+    lay, _ = U.layout()
+    o, _ = lay["bh"]
+    theta[o] = 50.0
+    net.set_params(theta)
+    net.set_odd(False)
+    dims = (40, 36, 30)
+    r = np.random.default_rng(9)
+    W = r.uniform(0.2, 1.0, int(np.prod(dims))).astype(np.float32)
+    wv = DeviceVector.from_array(net.dev, W)
+    zv = DeviceVector(net.dev, W.size)
+    try:
+        _, addr, user = net.precond(dims, overlap=8, split_weight=wv)
+        fn = PRECOND_FN(addr)
+        outs = []
+        for k in range(3):
+            g = r.standard_normal(W.size).astype(np.float32)
+            gv = DeviceVector.from_array(net.dev, g)
+            assert fn(user, gv.h, zv.h) == 0
+            outs.append((g, zv.numpy()))
+        assert net.guard_count() >= 2
+        g, z = outs[-1]
+        want = W.astype(np.float64) ** 2 * g
+        np.testing.assert_allclose(z, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    finally:
+        net.set_odd(True)
+        net.set_params(_theta(net))
