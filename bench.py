@@ -10,8 +10,8 @@ before the timed region.
 value = RTM Gpts/s = 2 * n1*n2*n3 * nt / time per shot (the two wavefields
 an RTM must advance; the checkpoint recompute is overhead, not credited).
 
-roofline: the dominant kernel is the fused VL stencil step (k_step_x4 at
-512^3: four cells per lane, no imaging; it runs for the source propagation
+roofline: the dominant kernel is the fused VL stencil step (k_step_x4r at
+512^3: four cells per lane on the six-plane halo ring, no imaging; it runs for the source propagation
 and the recompute).  Its launches inside the timed region are bracketed by
 HIP events on the library stream; achieved = 16 B per cell (read p_n,
 p_{n-1}, V; write p_{n+1}) * n1*n2*n3 / mean launch duration.  traffic is
@@ -36,10 +36,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 16.0
-# HBM bytes per launch of the plain VL step (k_step_x4, 512^3) from rocprofv3
+# HBM bytes per launch of the plain VL step (k_step_x4r, 512^3) from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration (2.0 / 1.0):
 # profiles/stencil512_traffic.md.  Used for the default configuration only.
-TRAFFIC_X4_512 = (2.0 * 922600 + 508032) * 1024.0
+TRAFFIC_X4R_512 = (2.0 * 798656 + 508032) * 1024.0
 
 
 def log(msg):
@@ -162,15 +162,17 @@ def main():
     k_ms = ktot / max(kcount, 1)
     achieved = BYTES_PER_CELL * ncell / (k_ms * 1e-3) / 1e9
     traffic = a.traffic_bytes or None
-    if traffic is None and w.n == (512, 512, 512) and a.x_per_lane in (0, 4) and a.ty == 0:
-        traffic = TRAFFIC_X4_512
+    default_cfg = w.n == (512, 512, 512) and a.x_per_lane in (0, 4) and a.ty == 0
+    if traffic is None and default_cfg:
+        traffic = TRAFFIC_X4R_512
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": {0: "brtm::k_step_x4 (VL, 4 cells/lane, fused sponge)",
+            "kernel": {0: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)",
                        1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
-                       4: "brtm::k_step_x4 (VL, 4 cells/lane, fused sponge)"}[a.x_per_lane],
+                       4: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)"
+                       }[a.x_per_lane],
             "kernel_ms": round(k_ms, 4), "launches": kcount,
             "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
     cpu = None

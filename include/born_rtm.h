@@ -177,9 +177,10 @@ int brtm_set_tuning(brtm_ctx *ctx, int ty_tile, int target_blocks);
 int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
 
 /* One- and four-cell VL steps staged through a ring of six LDS planes whose
- * halos are loaded together with the z-queue (on != 0), or through the
- * two-plane buffer that loads each halo one slice ahead (on = 0).  Same
- * results. */
+ * halos are loaded together with the z-queue (on = 1), or through the
+ * two-plane buffer that loads each halo one slice ahead (on = 0); -1, the
+ * default, uses the ring for the four-cell kernel (3 % faster at 512^3) and
+ * not for the one-cell kernel.  Same results. */
 int brtm_set_halo_ring(brtm_ctx *ctx, int on);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes

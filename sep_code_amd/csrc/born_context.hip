@@ -116,8 +116,10 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     if (want == 4 && c->n1 % 4 == 0) xl = 4;
     else if (want >= 2 && c->n1 % 2 == 0) xl = 2;
   }
+  // halo ring: automatic (-1) means on for the 4-cell kernel
+  const bool ring = c->halo_ring < 0 ? xl == 4 : c->halo_ring != 0;
   if (ty == 0) {
-    if (xl == 4) { ty = 16; tb = 256; }
+    if (xl == 4) { ty = (ring && act == ACT_IMG && fmt == FMT_F32) ? 8 : 16; tb = 256; }
     else if (xl == 2) { ty = 16; tb = 2048; }
     else if (act == ACT_NONE) { ty = 32; tb = 1024; }
     else if (fmt == FMT_F32) { ty = 16; tb = 8192; }
@@ -137,10 +139,10 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  if (xl == 1 && form == FORM_VL && c->halo_ring)
+  if (xl == 1 && form == FORM_VL && ring)
     BRTM_HIP(launch_step_ring(act, fmt, P, ty, c->stream));
   else if (xl == 4)
-    BRTM_HIP(launch_step_x4(act, fmt, P, ty, c->halo_ring, c->stream));
+    BRTM_HIP(launch_step_x4(act, fmt, P, ty, ring, c->stream));
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
@@ -807,7 +809,8 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
 
 int brtm_set_halo_ring(brtm_ctx *c, int on) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
-  c->halo_ring = on != 0;
+  BRTM_REQUIRE(on >= -1 && on <= 1, BRTM_EINVAL, "halo ring: -1 (automatic), 0 or 1");
+  c->halo_ring = on;
   return BRTM_OK;
 }
 

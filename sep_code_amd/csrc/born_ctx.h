@@ -72,7 +72,7 @@ struct brtm_ctx {
   float cimg = 1e-6f;
   int ty_tile = 0, target_blocks = 2048;  // ty_tile 0: chosen per step variant
   int x_per_lane = 0;  // VL steps: 0 automatic, 1, 2 (n1 even) or 4 (n1 % 4 == 0)
-  bool halo_ring = false;                  // 1-cell VL steps on the six-plane LDS ring
+  int halo_ring = -1;  // six-plane LDS ring: -1 automatic (on for 4 cells per lane), 0, 1
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)

@@ -185,8 +185,9 @@ class BornDevice:
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
     def set_halo_ring(self, on):
-        """One-cell VL steps on the six-plane LDS ring (halos loaded with the z-queue)."""
-        call("brtm_set_halo_ring", self.h, int(bool(on)))
+        """VL steps on the six-plane LDS ring (halos loaded with the z-queue):
+        1 on, 0 off, -1 automatic (the default: on for four cells per lane)."""
+        call("brtm_set_halo_ring", self.h, -1 if on is None or on == -1 else int(bool(on)))
 
     def set_x_per_lane(self, cells):
         """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1

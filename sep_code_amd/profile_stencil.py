@@ -10,7 +10,8 @@ kernels' counters), then --reps launches each of the stencil variants
 and prints their HIP-event mean durations and algorithmic bandwidths, for each
 cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 8-byte accesses; 4: 256-wide tiles with 16-byte accesses; names get an
-"_x2" / "_x4" suffix) and halo-ring setting in --ring (suffix "_ring").
+"_x2" / "_x4" suffix) and halo-ring setting in --ring (1 on, suffix "_ring";
+0 off, suffix "_noring"; -1 the library's automatic choice, no suffix).
 """
 from __future__ import annotations
 
@@ -34,7 +35,7 @@ def main():
     ap.add_argument("--ty", type=int, default=0)
     ap.add_argument("--variants", default="all")
     ap.add_argument("--xl", default="1,2")
-    ap.add_argument("--ring", default="0", help="comma list of halo-ring settings (0, 1)")
+    ap.add_argument("--ring", default="-1", help="comma list of halo-ring settings (-1, 0, 1)")
     a = ap.parse_args()
     n = a.n
     dev = BornDevice(0)
@@ -48,14 +49,13 @@ def main():
     for ring in [int(v) for v in a.ring.split(",")]:
         dev.set_halo_ring(ring)
         for xl in [int(v) for v in a.xl.split(",")]:
-            if ring and xl != 1:
-                continue
             dev.set_x_per_lane(xl)
             for name, form, act, fmt, B in VARIANTS:
                 if a.variants != "all" and name not in a.variants.split(","):
                     continue
                 ms = dev.time_step(form, act, fmt, a.reps)
-                key = name + (f"_x{xl}" if xl != 1 else "") + ("_ring" if ring else "")
+                key = name + (f"_x{xl}" if xl != 1 else "") + {1: "_ring", 0: "_noring",
+                                                                  -1: ""}[ring]
                 out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
                             "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)

@@ -102,7 +102,7 @@ def test_step_parity_variants(dev, n, ty, v):
         c2, _ = dev.propagate(0, 1, p1, p0)
         g2, g2o = dev.propagate(0, 60, p1, p0)
     finally:
-        set_variant(dev, (0, 0))
+        set_variant(dev, (0, -1))
         dev.set_tuning(0, 2048)
     o1, _ = P.propagate(0, 1, p1.ravel(), p0.ravel())
     o_cur, o_old = P.propagate(0, 60, p1.ravel(), p0.ravel())
@@ -199,7 +199,7 @@ def test_actions_variants(dev, store, v):
             dev.rtm_adjoint(1, P.nt, 2)
             out[k] = (D, dev.get_image())
         finally:
-            set_variant(dev, (0, 0))
+            set_variant(dev, (0, -1))
     base = out[(1, 0)]
     assert rel(out[v][0], base[0]) < 1e-6
     assert rel(out[v][1], base[1]) < 1e-6
