@@ -458,10 +458,13 @@ class CompositeBorn:
         nt = (nt_data - 1) * jt + 1
         self.ntr = nt_data + 8
         self.probs, self.snaps = [], []
+        # the operator's own jt-phase tables for the source and the data
+        # (Born/fd_prop_3d.cpp:28-32, Born/rtm_zero_op_3d.cpp:32-36)
+        sinc = sinc_table(jt) if jt > 1 else None
         for (sl, st, rl) in shots:
             P = Problem(n, d, V, bc_a, bc_b, src_locs=sl, src_tr=np.asarray(st, np.float64),
-                        rec_locs=mask_inactive(rl, n), ntr_d=self.ntr, jt=jt, nt=nt, jimg=jimg,
-                        cimg=cimg)
+                        sinc_s=sinc, rec_locs=mask_inactive(rl, n), ntr_d=self.ntr,
+                        sinc_d=sinc, jt=jt, nt=nt, jimg=jimg, cimg=cimg)
             self.probs.append(P)
             self.snaps.append(P.source_prop()[0])
 

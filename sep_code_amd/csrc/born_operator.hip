@@ -25,6 +25,7 @@ struct brtm_op {
   brtm_op_desc d{};
   std::vector<int> src_locs, rec_locs;
   std::vector<float> src_traces, mute;
+  std::vector<float> sinc;   // [jt][8] interpolation table for injection and extraction
   brtm::DevBuf d_mute;       // [m3][m2][m1]
   brtm::DevBuf work;         // compute-grid scratch
   int nt_fd = 0, ntr_d = 0;
@@ -107,6 +108,11 @@ int load_shot(brtm_op *op, int s) {
                          op->src_traces.data() + (size_t)s * d.nsrc * d.ntr_s));
   OP_TRY(brtm_set_receivers(c, d.nrx, d.nry, op->ntr_d, op->rec_locs.data() + (size_t)s * op->nrec(),
                             nullptr));
+  // The operator's own jt-phase tables for the source and the data
+  // (Born/fd_prop_3d.cpp:28-32, Born/rtm_zero_op_3d.cpp:32-36): installed
+  // for every shot because the context may be shared with other work.
+  OP_TRY(brtm_set_sinc_table(c, 0, 8, d.jt, op->sinc.data()));
+  OP_TRY(brtm_set_sinc_table(c, 1, 8, d.jt, op->sinc.data()));
   OP_TRY(brtm_select_shot(c, s));
   return BRTM_OK;
 }
@@ -246,6 +252,11 @@ int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out) {
   op->d.src_traces = nullptr;
   op->d.rec_locs = nullptr;
   op->d.mute = nullptr;
+  op->sinc.assign((size_t)8 * d.jt, 0.f);
+  if (brtm_sinc_table(d.jt, 8, op->sinc.data()) != BRTM_OK) {
+    delete op;
+    return BRTM_EINVAL;
+  }
   if (d.mute) {
     const size_t mb = sizeof(float) * (size_t)op->model_size();
     int rc = op->d_mute.alloc(mb);

@@ -217,6 +217,29 @@ def test_born_dot_product_fp64(jt, jimg):
     assert abs(lhs) > 0
 
 
+def test_composite_dot_product_two_steps_per_sample():
+    """The multi-shot operator (rtm_zero_op's composition D_t W B D_z E mute)
+    with two propagation steps per data sample is adjoint to its transpose,
+    and its sinc tables are the operator's own (delta at phase 0, the
+    half-sample interpolator at phase 1)."""
+    P, r = born_case(jt=2, nt=41)
+    n = P.n
+    shots = [(P.slocs, P.strc, P.rlocs), (P.slocs + 2, P.strc, P.rlocs)]
+    box = (n[0] - 10, n[1] - 10, n[2] - 12, 5, 5, 6)
+    mute = np.ones((box[2], box[1], box[0]))
+    mute[:2] = 0
+    C = O.CompositeBorn(n, (10.0, 10.0, 10.0), P.V.reshape(n[2], n[1], n[0]), 6.0, 0.03, shots,
+                        21, 0.002, jt=2, jimg=2, cimg=1e-3, model_box=box, mute=mute)
+    s = C.probs[0].sinc_d
+    assert abs(s[0, 3] - 1) < 1e-12 and np.allclose(s[1], s[1][::-1])
+    m = r.standard_normal(box[0] * box[1] * box[2])
+    d = r.standard_normal((2, len(P.rlocs), 21))
+    Fm = C.forward(m)
+    lhs, rhs = (Fm * d).sum(), (m * C.adjoint(d).ravel()).sum()
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+    assert abs(lhs) > 0
+
+
 def test_born_forward_live_source_equals_snapshots():
     P, r = born_case(jimg=2)
     snaps, _ = P.source_prop()

@@ -1,0 +1,180 @@
+"""SEP dataset interchange, parameter parsing and the command-line programs'
+host-side set-up (no GPU)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from sep_code_amd import geometry as G
+from sep_code_amd import rtm_cli, sep
+
+
+def test_round_trip_xdr_and_native(tmp_path):
+    r = np.random.default_rng(0)
+    a = r.standard_normal((3, 4, 5)).astype(np.float32)
+    axes = [sep.Axis(5, 0.0, 10.0, "x"), sep.Axis(4, -5.0, 12.5, "y"), sep.Axis(3, 1.0, 2.0)]
+    for fmt, dt in (("xdr_float", ">f4"), ("native_float", "<f4")):
+        h = str(tmp_path / f"a_{fmt}.H")
+        sep.write(h, a, axes, data_format=fmt)
+        b, bax = sep.read(h)
+        np.testing.assert_array_equal(a, b)
+        assert [(x.n, x.o, x.d, x.label) for x in bax] == \
+            [(x.n, x.o, x.d, x.label) for x in axes]
+        # the binary holds n1 fastest in the stated byte order
+        raw = np.fromfile(h + "@", dtype=dt)
+        np.testing.assert_array_equal(raw, a.ravel())
+
+
+def test_xdr_is_big_endian(tmp_path):
+    h = str(tmp_path / "one.H")
+    sep.write(h, np.array([1.0], np.float32), [sep.Axis(1)])
+    assert open(h + "@", "rb").read() == bytes([0x3F, 0x80, 0x00, 0x00])
+
+
+def test_hand_written_header(tmp_path, monkeypatch):
+    """A header as SEPlib programs write it: history lines, comments, quoted
+    values, a relative in= path, later assignments overriding earlier ones,
+    trailing unit axes dropped."""
+    v = np.arange(24, dtype=">f4")
+    v.tofile(str(tmp_path / "v.H@"))
+    (tmp_path / "v.H").write_text(
+        "Spike:  user@host   Mon Jan 1 00:00:00 2024\n"
+        "  n1=2 n2=3 n3=4 n4=1  o1=0 d1=10 # comment n5=7\n"
+        '  label1="x (m)" d2=20 d3=30 o3=-5\n'
+        "  esize=4 data_format=xdr_float in=v.H@\n"
+        "\n"
+        "Window: second history block\n"
+        "  d1=12.5\n")
+    monkeypatch.chdir(str(tmp_path.parent))
+    a, axes = sep.read(str(tmp_path / "v.H"))
+    assert a.shape == (4, 3, 2)
+    assert [x.n for x in axes] == [2, 3, 4]
+    assert axes[0].d == 12.5 and axes[0].label == "x (m)"
+    assert axes[2].o == -5.0 and axes[2].d == 30.0
+    np.testing.assert_array_equal(a.ravel(), np.arange(24))
+
+
+def test_header_errors(tmp_path):
+    (tmp_path / "bad.H").write_text("n1=4 in=missing.H@\n")
+    with pytest.raises(sep.SepError):
+        sep.read(str(tmp_path / "bad.H"))
+    (tmp_path / "e8.H").write_text("n1=1 esize=8 in=x\n")
+    with pytest.raises(sep.SepError):
+        sep.read(str(tmp_path / "e8.H"))
+    (tmp_path / "noin.H").write_text("n1=1\n")
+    with pytest.raises(sep.SepError):
+        sep.read(str(tmp_path / "noin.H"))
+    with pytest.raises(sep.SepError):
+        sep.write(str(tmp_path / "w.H"), np.zeros(3), [sep.Axis(4)])
+
+
+def test_params_par_files_and_defaults(tmp_path):
+    par = tmp_path / "p.par"
+    par.write_text("bc_a=20 # sponge\nniter=7\nvelocity=\"v.H\"\n")
+    p = sep.Params(["bc_a=10", f"par={par}", "niter=9", "junk"])
+    assert p.get_float("bc_a") == 20.0       # par= read where it appears
+    assert p.get_int("niter") == 9           # the last assignment wins
+    assert p.get_string("velocity") == "v.H"
+    assert p.get_float("bc_b", 0.0005) == 0.0005
+    assert p.get_int("nbound", int(p.get_float("bc_a"))) == 20
+    assert not p.has("mute")
+    with pytest.raises(sep.SepError):
+        p.get_string("data")
+
+
+def test_time_step_rule():
+    # a 4 ms data interval, 3000 m/s, 10 m cells: the limit 1.5 ms gives
+    # three steps per sample
+    dt, jt = rtm_cli.time_step(0.004, 3000.0, 10.0)
+    assert jt == 3 and math.isclose(dt, 0.004 / 3)
+    assert 3000.0 * dt / 10.0 <= rtm_cli.COURANT
+    # the interval itself when it is stable; exact multiples are not split
+    assert rtm_cli.time_step(0.001, 3000.0, 10.0) == (0.001, 1)
+    assert rtm_cli.time_step(0.003, 3000.0, 10.0)[1] == 2
+
+
+def test_pad_velocity_repeats_the_nearest_model_sample():
+    r = np.random.default_rng(1)
+    m = (5, 4, 3)
+    vel = r.uniform(1500, 3000, (m[2], m[1], m[0])).astype(np.float32)
+    axes = [sep.Axis(5, 0.0, 10.0), sep.Axis(4, 0.0, 10.0), sep.Axis(3, 0.0, 10.0)]
+    nbound = 6
+    n = (16, 16, 32)
+    vc = rtm_cli.pad_velocity(vel, axes, n, nbound)
+    assert vc.shape == (32, 16, 16)
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                ii = min(max(i - nbound, 0), m[0] - 1)
+                jj = min(max(j - nbound, 0), m[1] - 1)
+                kk = min(max(k - nbound, 0), m[2] - 1)
+                assert vc[k, j, i] == vel[kk, jj, ii]
+
+
+def write_small_survey(tmp_path, m=(24, 20, 16), d=10.0, nt=60, dt_data=0.002,
+                       shots=((40.0, 80.0, 2), (50.0, 50.0, 1)), rec=((20.0, 40.0, 5),
+                                                                      (30.0, 50.0, 3))):
+    """velocity.H, wavelet.H and a data.H header of a small survey."""
+    n1, n2, n3 = m
+    z = np.arange(n3)[:, None, None]
+    vel = np.where(z < n3 // 2, 1800.0, 2600.0) * np.ones((n3, n2, n1))
+    vaxes = [sep.Axis(n1, 0.0, d, "x"), sep.Axis(n2, 0.0, d, "y"), sep.Axis(n3, 0.0, d, "z")]
+    sep.write(str(tmp_path / "vel.H"), vel, vaxes)
+    w = G.ricker(25.0, dt_data, nt)
+    sep.write(str(tmp_path / "wav.H"), w, [sep.Axis(nt, 0.0, dt_data, "t")])
+    (sx_o, sx_d, sx_n), (sy_o, sy_d, sy_n) = shots
+    (rx_o, rx_d, rx_n), (ry_o, ry_d, ry_n) = rec
+    daxes = [sep.Axis(nt, 0.0, dt_data, "t"), sep.Axis(rx_n, rx_o, rx_d, "rx"),
+             sep.Axis(ry_n, ry_o, ry_d, "ry"), sep.Axis(sx_n, sx_o, sx_d, "sx"),
+             sep.Axis(sy_n, sy_o, sy_d, "sy")]
+    sep.write(str(tmp_path / "data.H"), np.zeros([a.n for a in reversed(daxes)]), daxes)
+    return vel.astype(np.float32), vaxes, daxes
+
+
+def test_problem_from_sep_geometry(tmp_path):
+    vel, vaxes, daxes = write_small_survey(tmp_path)
+    p = sep.Params([f"velocity={tmp_path}/vel.H", f"wavelet={tmp_path}/wav.H",
+                    f"data={tmp_path}/data.H", "bc_a=8", "src_depth=20", "rec_depth=10"])
+    sp = rtm_cli.problem_from_sep(p, "f32")
+    w = sp.workload
+    nbound = 8 + 4
+    # padded by nbound each side (plus nb + nb + 2 fat cells below), rounded to 16
+    assert w.n == (48, 48, 64)
+    assert w.nbound == nbound
+    assert sp.model_box == (24, 20, 16, nbound, nbound, nbound)
+    # 2600 m/s, 10 m cells, 2 ms data: limit 1.73 ms -> two steps of 1 ms
+    assert w.jt == 2 and math.isclose(w.dt, 0.001) and w.nt == 60
+    assert w.ntr == 68
+    # shots: x fastest over the (sx, sy) grid (Born/source_func_3d.cpp:21-31)
+    assert len(w.shots) == 2
+    n1, n2 = w.n[0], w.n[1]
+    for s, (sx, sy) in zip(w.shots, [(40.0, 50.0), (120.0, 50.0)]):
+        fx, fy, fz = sx / 10 + nbound, sy / 10 + nbound, 20 / 10 + nbound
+        cells = {int(c) for c in s.src_locs}
+        iz = int(fz) - 1
+        expect = {i1 + i2 * n1 + iz * n1 * n2 for i2 in range(int(fy) - 1, int(fy) + 2)
+                  for i1 in range(int(fx) - 1, int(fx) + 2)}
+        assert cells == expect
+        # the wavelet's derivative in samples 4..; the centre cell sits one
+        # plane above the source depth: weight exp(-1)
+        wd = G.wavelet_derivative(G.ricker(25.0, 0.002, 60), 0.002)
+        np.testing.assert_allclose(s.src_traces[4, 4:64], math.exp(-1) * wd, rtol=1e-5,
+                                   atol=1e-3)
+        # receivers: rx fastest, depth cell int(rec_depth/d + nbound) + 4
+        assert s.nrx == 5 and s.nry == 3
+        rx = 20.0 + 40.0 * np.arange(5)
+        ry = 30.0 + 50.0 * np.arange(3)
+        exp = [int(x / 10 + nbound) + int(y / 10 + nbound) * n1 + (1 + nbound + 4) * n1 * n2
+               for y in ry for x in rx]
+        np.testing.assert_array_equal(s.rec_locs, exp)
+    # velocity: model interior copied, padding repeats the edge
+    vc = w.vel
+    np.testing.assert_array_equal(vc[nbound:nbound + 16, nbound:nbound + 20, nbound:nbound + 24],
+                                  vel)
+    assert (vc[:nbound] == 1800.0).all() and (vc[nbound + 16:] == 2600.0).all()
+
+
+def test_cli_usage_without_program():
+    assert rtm_cli.main([]) == 2
+    assert rtm_cli.main(["migrate"]) == 2
