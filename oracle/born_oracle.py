@@ -17,6 +17,9 @@ operators that surround the propagation in the reference:
 * ``sinc_table``              -- Born/sinc_bob.cpp:4-114 (Hale's
   least-squares tapered sinc, Levinson solve) in FP64;
 * ``cgstep``                  -- Born/cgstep.cpp:14-58;
+* ``lin_solve``               -- simple_solver / reg_solver / prec_solver,
+  Born/lin_solver.cpp:7-53,79-160 (stacked systems of Born/combo_oper.cpp);
+* ``tmute_start/tmute_apply`` -- Born/tmute.cpp:4-76;
 * ``source_geometry``         -- Born/source_func_3d.cpp:100-133;
 * ``receiver_index``          -- Born/map_data_3d.cpp:4-32.
 
@@ -392,6 +395,90 @@ def fcg_solve(A, d, niter, history, precond=None):
         res.append(rr @ rr)
         dirs.append(z)
     return m, res, dirs
+
+
+def _ops(A):
+    """(forward, adjoint) callables of a dense matrix or a callable pair."""
+    if callable(A) or isinstance(A, tuple):
+        return A
+    Ad = np.asarray(A, np.float64)
+    return (lambda x: Ad @ x), (lambda y: Ad.T @ y)
+
+
+def lin_solve(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    """simple_solver / reg_solver / prec_solver (Born/lin_solver.cpp:7-53,
+    79-101,115-160) in FP64, on operators given as dense matrices or
+    (forward, adjoint) pairs:
+
+    * "simple": min |W(L m - d)|;
+    * "reg":    min |W(L m - d)|^2 + |eps R m|^2, R = aux (col_op of W L and
+                the scaled R, Born/lin_solver.cpp:39-53);
+    * "prec":   m = P p, min |W(L P p - d)|^2 + |eps p|^2, P = aux (col_op of
+                the chain W L P and eps I, Born/lin_solver.cpp:21-37).
+
+    With m0 the solver works on the update: the initial residual is
+    W(L m0 - d) (init_rr, Born/lin_solver.cpp:79-101) and the result is the
+    update.  The iteration is fcg_solve on the stacked system with the
+    stacked right-hand side [W(d - L m0); 0].  Returns (model, [rr.rr])."""
+    Lf, La = _ops(L)
+    Wf, Wa = _ops(W) if W is not None else ((lambda y: y), (lambda y: y))
+    d = np.asarray(d, np.float64).ravel()
+    r = d - Lf(np.asarray(m0, np.float64).ravel()) if m0 is not None else d
+    rhs = Wf(r)
+    if kind == "simple":
+        fwd = lambda m: Wf(Lf(m))  # noqa: E731
+        adj = lambda y: La(Wa(y))  # noqa: E731
+        m, res, _ = fcg_solve((fwd, adj), rhs, niter, history)
+        return m, res
+    Af, Aa = _ops(aux)
+    nr = rhs.size
+    if kind == "reg":
+        def fwd(m):
+            return np.concatenate([Wf(Lf(m)), eps * Af(m)])
+
+        def adj(y):
+            return La(Wa(y[:nr])) + eps * Aa(y[nr:])
+
+        nm = La(Wa(np.zeros(nr))).size
+        nextra = Af(np.zeros(nm)).size
+        m, res, _ = fcg_solve((fwd, adj), np.concatenate([rhs, np.zeros(nextra)]), niter,
+                              history)
+        return m, res
+    if kind == "prec":
+        def fwd(p):
+            return np.concatenate([Wf(Lf(Af(p))), eps * p])
+
+        def adj(y):
+            p = Aa(La(Wa(y[:nr])))
+            return p + eps * y[nr:]
+
+        npd = Aa(La(Wa(np.zeros(nr)))).size
+        p, res, _ = fcg_solve((fwd, adj), np.concatenate([rhs, np.zeros(npd)]), niter, history)
+        return Af(p), res
+    raise ValueError(kind)
+
+
+def tmute_start(nt, ot, dt, nrx, orx, drx, t0, vmute):
+    """First kept sample of each receiver column of tmute (Born/tmute.cpp:
+    57-59): float t0 + (o2 + d2*i2)/vmute, truncated (t - o1)/d1, clamped to
+    [0, nt] (the reference indexes from a negative start when the mute time
+    precedes o1)."""
+    f32 = np.float32
+    out = np.zeros(nrx, np.int64)
+    for i in range(nrx):
+        t = f32(t0) + (f32(orx) + f32(drx) * f32(i)) / f32(vmute)
+        f = (f32(t) - f32(ot)) / f32(dt)
+        out[i] = 0 if f <= 0 else (nt if f >= nt else int(f))
+    return out
+
+
+def tmute_apply(x, start):
+    """tmute forward (= adjoint) on data [..., nrx, nt]: zero the samples
+    before start[rx]."""
+    x = np.asarray(x, np.float64)
+    nt = x.shape[-1]
+    keep = np.arange(nt)[None, :] >= np.asarray(start)[:, None]
+    return x * keep
 
 
 def source_geometry(n1, n2, fz, fx, fy, wavelet_deriv, nts):

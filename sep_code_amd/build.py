@@ -20,8 +20,8 @@ INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 LIBNAME = "libborn_rtm.so"
 ARCH = os.environ.get("BRTM_ARCH", "gfx950")
 
-SOURCES = ["born_kernels.hip", "born_context.hip", "born_operator.hip", "born_compat.hip",
-           "conv3d.hip", "unet.hip"]
+SOURCES = ["born_kernels.hip", "born_context.hip", "born_operator.hip", "born_solver.hip",
+           "born_compat.hip", "conv3d.hip", "unet.hip"]
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -119,3 +119,23 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
          const void *snap, const float *sscale, float *img_or_null, const float *m_or_null);
 
 }  // namespace brtm
+
+// A float vector in HBM (the C ABI's brtm_vec).  ptr is the storage: buf's
+// allocation for an owning vector, or a slice of another vector's storage
+// for a view (a part of a stacked vector), which owns nothing.
+struct brtm_vec {
+  brtm_ctx *ctx = nullptr;
+  long n = 0;
+  float *ptr = nullptr;
+  brtm::DevBuf buf;
+  float *p() const { return ptr; }
+};
+
+struct brtm_op;
+
+namespace brtm {
+// BRTM_EINVAL with a message unless v is a vector of n elements.
+int check_vec(const brtm_vec *v, long n, const char *what);
+// The context a Born operator propagates on.
+brtm_ctx *op_context(const brtm_op *op);
+}  // namespace brtm

@@ -13,13 +13,6 @@
 #include "../../include/born_rtm.h"
 #include "born_ctx.h"
 
-struct brtm_vec {
-  brtm_ctx *ctx = nullptr;
-  long n = 0;
-  brtm::DevBuf buf;
-  float *p() const { return buf.as<float>(); }
-};
-
 struct brtm_op {
   brtm_ctx *ctx = nullptr;
   brtm_op_desc d{};
@@ -36,6 +29,17 @@ struct brtm_op {
 };
 
 namespace brtm {
+
+int check_vec(const brtm_vec *v, long n, const char *what) {
+  BRTM_REQUIRE(v && v->p(), BRTM_EINVAL, std::string(what) + " is NULL");
+  BRTM_REQUIRE(v->n == n, BRTM_EINVAL,
+               std::string(what) + " has " + std::to_string(v->n) + " elements, expected " +
+                   std::to_string(n));
+  return BRTM_OK;
+}
+
+brtm_ctx *op_context(const brtm_op *op) { return op ? op->ctx : nullptr; }
+
 namespace {
 
 __global__ void k_random(float *v, long n, unsigned long long seed) {
@@ -86,13 +90,6 @@ void levinson(int m, const double *r, const double *g, double *f) {
   }
 }
 
-int check_vec(const brtm_vec *v, long n, const char *what) {
-  BRTM_REQUIRE(v && v->buf.ptr, BRTM_EINVAL, std::string(what) + " is NULL");
-  BRTM_REQUIRE(v->n == n, BRTM_EINVAL,
-               std::string(what) + " has " + std::to_string(v->n) + " elements, expected " +
-                   std::to_string(n));
-  return BRTM_OK;
-}
 
 #define OP_TRY(x)                     \
   do {                                \
@@ -136,7 +133,8 @@ int brtm_vec_create(brtm_ctx *ctx, long n, brtm_vec **out) {
     delete v;
     return rc;
   }
-  hipError_t e = hipMemsetAsync(v->buf.ptr, 0, sizeof(float) * (size_t)n, ctx->stream);
+  v->ptr = v->buf.as<float>();
+  hipError_t e = hipMemsetAsync(v->p(), 0, sizeof(float) * (size_t)n, ctx->stream);
   if (e != hipSuccess) {
     delete v;
     set_error(hipGetErrorString(e));
@@ -154,11 +152,11 @@ int brtm_vec_destroy(brtm_vec *v) {
 }
 
 long brtm_vec_size(const brtm_vec *v) { return v ? v->n : 0; }
-void *brtm_vec_data(brtm_vec *v) { return v ? v->buf.ptr : nullptr; }
+void *brtm_vec_data(brtm_vec *v) { return v ? v->p() : nullptr; }
 
 int brtm_vec_upload(brtm_vec *v, const float *host) {
   BRTM_REQUIRE(v && host, BRTM_EINVAL, "NULL argument");
-  BRTM_HIP(hipMemcpyAsync(v->buf.ptr, host, sizeof(float) * (size_t)v->n, hipMemcpyHostToDevice,
+  BRTM_HIP(hipMemcpyAsync(v->p(), host, sizeof(float) * (size_t)v->n, hipMemcpyHostToDevice,
                           v->ctx->stream));
   BRTM_HIP(hipStreamSynchronize(v->ctx->stream));
   return BRTM_OK;
@@ -166,7 +164,7 @@ int brtm_vec_upload(brtm_vec *v, const float *host) {
 
 int brtm_vec_download(const brtm_vec *v, float *host) {
   BRTM_REQUIRE(v && host, BRTM_EINVAL, "NULL argument");
-  BRTM_HIP(hipMemcpyAsync(host, v->buf.ptr, sizeof(float) * (size_t)v->n, hipMemcpyDeviceToHost,
+  BRTM_HIP(hipMemcpyAsync(host, v->p(), sizeof(float) * (size_t)v->n, hipMemcpyDeviceToHost,
                           v->ctx->stream));
   BRTM_HIP(hipStreamSynchronize(v->ctx->stream));
   return BRTM_OK;
@@ -174,13 +172,13 @@ int brtm_vec_download(const brtm_vec *v, float *host) {
 
 int brtm_vec_zero(brtm_vec *v) {
   BRTM_REQUIRE(v, BRTM_EINVAL, "NULL vector");
-  BRTM_HIP(hipMemsetAsync(v->buf.ptr, 0, sizeof(float) * (size_t)v->n, v->ctx->stream));
+  BRTM_HIP(hipMemsetAsync(v->p(), 0, sizeof(float) * (size_t)v->n, v->ctx->stream));
   return BRTM_OK;
 }
 
 int brtm_vec_copy(brtm_vec *dst, const brtm_vec *src) {
   BRTM_REQUIRE(dst && src && dst->n == src->n, BRTM_EINVAL, "vector size mismatch");
-  BRTM_HIP(hipMemcpyAsync(dst->buf.ptr, src->buf.ptr, sizeof(float) * (size_t)src->n,
+  BRTM_HIP(hipMemcpyAsync(dst->p(), src->p(), sizeof(float) * (size_t)src->n,
                           hipMemcpyDeviceToDevice, dst->ctx->stream));
   return BRTM_OK;
 }
@@ -308,7 +306,7 @@ int brtm_op_forward(brtm_op *op, int add, const brtm_vec *model, brtm_vec *data)
   const brtm_op_desc &d = op->d;
   BRTM_HIP(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  if (!add) BRTM_HIP(hipMemsetAsync(data->buf.ptr, 0, sizeof(float) * (size_t)data->n, st));
+  if (!add) BRTM_HIP(hipMemsetAsync(data->p(), 0, sizeof(float) * (size_t)data->n, st));
   // m~ = D_z E (mute . m): the same for every shot.
   float *w = op->work.as<float>();
   const float *msrc = model->p();
@@ -342,7 +340,7 @@ int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data)
   const brtm_op_desc &d = op->d;
   BRTM_HIP(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  if (!add) BRTM_HIP(hipMemsetAsync(model->buf.ptr, 0, sizeof(float) * (size_t)model->n, st));
+  if (!add) BRTM_HIP(hipMemsetAsync(model->p(), 0, sizeof(float) * (size_t)model->n, st));
   BRTM_HIP(hipMemsetAsync(c->img.ptr, 0, sizeof(float) * (size_t)c->ncell, st));
   const float inv_dt = 1.f / d.dt_data;
   const long nrec = op->nrec();
@@ -379,7 +377,7 @@ int brtm_op_illumination(brtm_op *op, brtm_vec *out) {
   const brtm_op_desc &d = op->d;
   BRTM_HIP(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  BRTM_HIP(hipMemsetAsync(out->buf.ptr, 0, sizeof(float) * (size_t)out->n, st));
+  BRTM_HIP(hipMemsetAsync(out->p(), 0, sizeof(float) * (size_t)out->n, st));
   float *acc = op->work.as<float>();
   BRTM_HIP(hipMemsetAsync(acc, 0, sizeof(float) * (size_t)c->ncell, st));
   for (int s = 0; s < d.nshots; ++s) {
@@ -423,278 +421,6 @@ int brtm_op_dot_test(brtm_op *op, unsigned long long seed, double err[2]) {
   err[0] = std::fabs((dot1 - dot2) / dot1);
   err[1] = std::fabs((dot3 - dot4) / dot3);
   return BRTM_OK;
-}
-
-// ------------------------------------------------------------------- CG
-namespace {
-
-// Solve the K x K symmetric system M x = b (FP64) by Cholesky, skipping
-// basis vectors whose pivot falls below tol times their own norm (nearly
-// dependent on the ones before them): their coefficient is 0.  Returns the
-// number of basis vectors kept (0 when the first one has no length).
-int gram_solve(int K, const double *M, const double *b, double *x, double tol = 1e-9) {
-  double Lm[BRTM_CG_MAX_HISTORY + 1][BRTM_CG_MAX_HISTORY + 1] = {};
-  bool keep[BRTM_CG_MAX_HISTORY + 1] = {};
-  int kept = 0;
-  for (int i = 0; i < K; ++i) {
-    double d = M[i * K + i];
-    for (int k = 0; k < i; ++k)
-      if (keep[k]) d -= Lm[i][k] * Lm[i][k];
-    if (!(d > tol * M[i * K + i]) || !(M[i * K + i] > 0)) {
-      keep[i] = false;
-      continue;
-    }
-    keep[i] = true;
-    ++kept;
-    Lm[i][i] = std::sqrt(d);
-    for (int j = i + 1; j < K; ++j) {
-      double v = M[j * K + i];
-      for (int k = 0; k < i; ++k)
-        if (keep[k]) v -= Lm[j][k] * Lm[i][k];
-      Lm[j][i] = v / Lm[i][i];
-    }
-  }
-  double y[BRTM_CG_MAX_HISTORY + 1] = {};
-  for (int i = 0; i < K; ++i) {
-    if (!keep[i]) continue;
-    double v = b[i];
-    for (int k = 0; k < i; ++k)
-      if (keep[k]) v -= Lm[i][k] * y[k];
-    y[i] = v / Lm[i][i];
-  }
-  for (int i = K - 1; i >= 0; --i) {
-    x[i] = 0;
-    if (!keep[i]) continue;
-    double v = y[i];
-    for (int k = i + 1; k < K; ++k)
-      if (keep[k]) v -= Lm[k][i] * x[k];
-    x[i] = v / Lm[i][i];
-  }
-  return kept;
-}
-
-}  // namespace
-
-int brtm_cg_solve(brtm_op *op, const brtm_vec *data, brtm_vec *model,
-                  const brtm_cg_params *prm, double *resid_hist, double *time_hist,
-                  brtm_cg_stats *stats) {
-  BRTM_REQUIRE(op && data && model && prm, BRTM_EINVAL, "NULL argument");
-  OP_TRY(check_vec(model, op->model_size(), "model"));
-  OP_TRY(check_vec(data, op->data_size(), "data"));
-  brtm_ctx *c = op->ctx;
-  brtm_cg_stats S{};
-  const long nm = op->model_size(), nd = op->data_size();
-  BRTM_REQUIRE(prm->history >= 0 && prm->history <= BRTM_CG_MAX_HISTORY, BRTM_EINVAL,
-               "history must be in [0, BRTM_CG_MAX_HISTORY]");
-  const int H = prm->history >= 2 ? prm->history : 0;  // 0: cgstep
-  brtm_vec *g = nullptr, *z = nullptr, *s = nullptr, *rr = nullptr, *gg = nullptr, *ss = nullptr;
-  // flexible CG: the last H steps in model (hs) and data (hS) space, newest first
-  std::vector<brtm_vec *> hs(H, nullptr), hS(H, nullptr);
-  int rc = brtm_vec_create(c, nm, &g);
-  if (rc == BRTM_OK) rc = brtm_vec_create(c, nm, &s);
-  if (rc == BRTM_OK && prm->precond) rc = brtm_vec_create(c, nm, &z);
-  if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &rr);
-  if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &gg);
-  if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &ss);
-  for (int i = 0; i < H && rc == BRTM_OK; ++i) {
-    rc = brtm_vec_create(c, nm, &hs[i]);
-    if (rc == BRTM_OK) rc = brtm_vec_create(c, nd, &hS[i]);
-  }
-  int nh = 0;  // steps held
-  auto cleanup = [&]() {
-    for (brtm_vec *v : {g, z, s, rr, gg, ss}) brtm_vec_destroy(v);
-    for (int i = 0; i < H; ++i) {
-      brtm_vec_destroy(hs[i]);
-      brtm_vec_destroy(hS[i]);
-    }
-  };
-  if (rc != BRTM_OK) {
-    cleanup();
-    return rc;
-  }
-  const auto t0 = std::chrono::steady_clock::now();
-  auto elapsed = [&]() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  };
-  // init_rr (Born/lin_solver.cpp:79-101) with m0 = 0: rr = -d; m = 0.
-  double dd = 0;
-  rc = brtm_vec_zero(model);
-  if (rc == BRTM_OK) rc = brtm_vec_copy(rr, data);
-  if (rc == BRTM_OK) rc = brtm_vec_scale(rr, -1.0);
-  if (rc == BRTM_OK) rc = brtm_vec_dot(data, data, &dd);
-  if (rc != BRTM_OK) {
-    cleanup();
-    return rc;
-  }
-  const double dnorm = std::sqrt(dd);
-  // Judged residual: |rr + e| / |d - e| with the optional offset e.
-  const brtm_vec *e = prm->offset;
-  double ee = 0, de = 0;
-  if (e) {
-    rc = check_vec(e, nd, "offset");
-    if (rc == BRTM_OK) rc = brtm_vec_dot(e, e, &ee);
-    if (rc == BRTM_OK) rc = brtm_vec_dot(data, e, &de);
-    if (rc != BRTM_OK) {
-      cleanup();
-      return rc;
-    }
-  }
-  const double jnorm = e ? std::sqrt(std::max(dd - 2 * de + ee, 0.0)) : dnorm;
-  auto judged = [&](double rr_rr, double *out) -> int {
-    if (!e) {
-      *out = std::sqrt(rr_rr) / dnorm;
-      return BRTM_OK;
-    }
-    double re = 0;
-    int r2 = brtm_vec_dot(rr, e, &re);
-    *out = std::sqrt(std::max(rr_rr + 2 * re + ee, 0.0)) / jnorm;
-    return r2;
-  };
-  S.rel_resid = 1.0;
-  if (e) {
-    rc = judged(dd, &S.rel_resid);  // rr = -d at m = 0
-    if (rc != BRTM_OK) {
-      cleanup();
-      return rc;
-    }
-  }
-  if (resid_hist) resid_hist[0] = S.rel_resid;
-  if (time_hist) time_hist[0] = 0.0;
-  bool first = true;  // cgstep::forget (Born/cgstep.cpp:16-26)
-  const double small = 1e-20;
-  for (int iter = 0; iter < prm->niter && rc == BRTM_OK; ++iter) {
-    if (dnorm == 0) {
-      S.breakdown = 1;
-      break;
-    }
-    // g = L^T rr ; direction = P(g) (or g) ; gg = L direction
-    rc = brtm_op_adjoint(op, 0, g, rr);
-    if (rc != BRTM_OK) break;
-    brtm_vec *dir = g;
-    if (prm->precond) {
-      if (prm->precond(prm->precond_user, g, z) != 0) {
-        set_error("preconditioner failed");
-        rc = BRTM_ESTATE;
-        break;
-      }
-      double zg = 0;
-      rc = brtm_vec_dot(z, g, &zg);
-      if (rc != BRTM_OK) break;
-      if (zg > 0 && std::isfinite(zg)) dir = z;  // descent direction for |Lm-d|^2/2 is -g
-      else ++S.fallbacks;
-    }
-    rc = brtm_op_forward(op, 0, dir, gg);
-    if (rc != BRTM_OK) break;
-    double alfa, beta;
-    if (H) {
-      // Flexible CG: minimise |rr + a gg + sum_i b_i hS_i| over (a, b).
-      const int K = nh + 1;
-      const brtm_vec *B[BRTM_CG_MAX_HISTORY + 1];
-      B[0] = gg;
-      for (int i = 0; i < nh; ++i) B[i + 1] = hS[i];
-      double M[(BRTM_CG_MAX_HISTORY + 1) * (BRTM_CG_MAX_HISTORY + 1)], bv[BRTM_CG_MAX_HISTORY + 1],
-          x[BRTM_CG_MAX_HISTORY + 1];
-      for (int a = 0; a < K && rc == BRTM_OK; ++a) {
-        for (int b = 0; b <= a && rc == BRTM_OK; ++b) {
-          rc = brtm_vec_dot(B[a], B[b], &M[a * K + b]);
-          M[b * K + a] = M[a * K + b];
-        }
-        if (rc == BRTM_OK) rc = brtm_vec_dot(B[a], rr, &bv[a]);
-        bv[a] = -bv[a];
-      }
-      if (rc != BRTM_OK) break;
-      if (std::fabs(M[0]) < small || gram_solve(K, M, bv, x) == 0) {
-        S.breakdown = 1;
-        break;
-      }
-      // new step into the scratch pair (s, ss)
-      rc = brtm_vec_copy(s, dir);
-      if (rc == BRTM_OK) rc = brtm_vec_scale(s, x[0]);
-      if (rc == BRTM_OK) rc = brtm_vec_copy(ss, gg);
-      if (rc == BRTM_OK) rc = brtm_vec_scale(ss, x[0]);
-      for (int i = 0; i < nh && rc == BRTM_OK; ++i) {
-        if (x[i + 1] == 0) continue;
-        rc = brtm_vec_scale_add(s, 1.0, hs[i], x[i + 1]);
-        if (rc == BRTM_OK) rc = brtm_vec_scale_add(ss, 1.0, hS[i], x[i + 1]);
-      }
-      if (rc == BRTM_OK) rc = brtm_vec_add(model, s);
-      if (rc == BRTM_OK) rc = brtm_vec_add(rr, ss);
-      if (rc != BRTM_OK) break;
-      // push: the new step becomes hs[0]; the dropped (or unused) slot
-      // becomes the next scratch pair
-      const int slot = nh < H ? nh : H - 1;
-      std::swap(s, hs[slot]);
-      std::swap(ss, hS[slot]);
-      std::rotate(hs.begin(), hs.begin() + slot, hs.begin() + slot + 1);
-      std::rotate(hS.begin(), hS.begin() + slot, hS.begin() + slot + 1);
-      nh = std::min(nh + 1, H);
-      alfa = x[0];
-      beta = nh > 1 ? x[1] : 0.0;
-    } else {
-      // cgstep::step_it (Born/cgstep.cpp:14-58)
-      double gg_gg = 0;
-      rc = brtm_vec_dot(gg, gg, &gg_gg);
-      if (rc != BRTM_OK) break;
-      if (first) {
-        if (std::fabs(gg_gg) < small) {
-          S.breakdown = 1;
-          break;
-        }
-        double gg_rr = 0;
-        rc = brtm_vec_dot(gg, rr, &gg_rr);
-        if (rc != BRTM_OK) break;
-        beta = 0;
-        alfa = -gg_rr / gg_gg;
-      } else {
-        double ss_ss = 0, gg_ss = 0, gg_rr = 0, ss_rr = 0;
-        rc = brtm_vec_dot(ss, ss, &ss_ss);
-        if (rc == BRTM_OK) rc = brtm_vec_dot(gg, ss, &gg_ss);
-        if (rc == BRTM_OK) rc = brtm_vec_dot(gg, rr, &gg_rr);
-        if (rc == BRTM_OK) rc = brtm_vec_dot(ss, rr, &ss_rr);
-        if (rc != BRTM_OK) break;
-        if (std::fabs(gg_gg) < small || std::fabs(ss_ss) < small) {
-          S.breakdown = 1;
-          break;
-        }
-        const double determ = gg_gg * ss_ss * (1.0 - gg_ss / gg_gg * gg_ss / ss_ss);
-        gg_rr = -gg_rr;
-        ss_rr = -ss_rr;
-        alfa = (ss_ss * gg_rr - gg_ss * ss_rr) / determ;
-        beta = (-gg_ss * gg_rr + gg_gg * ss_rr) / determ;
-      }
-      rc = brtm_vec_scale_add(s, beta, dir, alfa);       // s  = beta s  + alfa g
-      if (rc == BRTM_OK) rc = brtm_vec_scale_add(ss, beta, gg, alfa);  // ss = beta ss + alfa gg
-      if (rc == BRTM_OK) rc = brtm_vec_add(model, s);                  // m += s
-      if (rc == BRTM_OK) rc = brtm_vec_add(rr, ss);                    // rr += ss
-    }
-    double rr_rr = 0;
-    if (rc == BRTM_OK) rc = brtm_vec_dot(rr, rr, &rr_rr);
-    if (rc != BRTM_OK) break;
-    first = false;
-    S.iterations = iter + 1;
-    rc = judged(rr_rr, &S.rel_resid);
-    if (rc != BRTM_OK) break;
-    if (resid_hist) resid_hist[iter + 1] = S.rel_resid;
-    if (time_hist) time_hist[iter + 1] = elapsed();
-    if (prm->verbose)
-      std::fprintf(stderr, "cg iter %d alpha=%g beta=%g rel_resid=%.6e t=%.2fs\n", iter, alfa,
-                   beta, S.rel_resid, elapsed());
-    if (prm->rel_tol > 0 && S.rel_resid <= prm->rel_tol) {
-      S.converged = 1;
-      break;
-    }
-  }
-  if (rc == BRTM_OK) {
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-      set_error(hipGetErrorString(e));
-      rc = BRTM_EHIP;
-    }
-  }
-  S.seconds = elapsed();
-  cleanup();
-  if (stats) *stats = S;
-  return rc;
 }
 
 int brtm_sinc_table(int ntab, int nsinc, float *table) {

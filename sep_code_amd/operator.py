@@ -6,10 +6,17 @@ include/born_op.h.
 * ``RtmZeroOp``     -- rtm_zero_op (Born/rtm_zero_op_3d.h:9-38):
   ``forward(add, model, data)`` / ``adjoint(add, model, data)`` /
   ``dot_test()``;
-* ``cgstep`` / ``SimpleSolver`` -- Born/cgstep.h, Born/lin_solver.h:
-  ``SimpleSolver(cgstep(), data, op).solve(niter)`` runs the CG of
-  Born/lin_solver.cpp:115-143 on the device, with an optional relative
-  data-residual stop and an optional preconditioner of the gradient.
+* ``LinOp`` and its kinds (include/born_solver.h) -- my_operator:
+  ``BornOp`` (rtm_zero_op), ``IOp`` (i_op), ``WeightOp`` (pointwise
+  weight), ``TMute`` (tmute), ``CallbackOp`` (a Python operator),
+  ``RowOp`` / ``ColOp`` / ``DiagOp`` (row_op / col_op / diag_op,
+  Born/combo_oper.cpp);
+* ``cgstep`` / ``SimpleSolver`` / ``RegSolver`` / ``PrecSolver`` --
+  Born/cgstep.h, Born/lin_solver.h: ``SimpleSolver(cgstep(), data,
+  op).solve(niter)`` runs the CG of Born/lin_solver.cpp:115-143 on the
+  device, with an optional relative residual stop and an optional
+  preconditioner of the gradient; the regularised and preconditioned forms
+  stack their rows as the reference's col_op does.
 """
 from __future__ import annotations
 
@@ -35,6 +42,7 @@ class OpDesc(C.Structure):
 
 
 PRECOND_FN = C.CFUNCTYPE(_I, _P, _P, _P)
+APPLY_FN = C.CFUNCTYPE(_I, _P, _I, _P, _P)
 
 
 class CgParams(C.Structure):
@@ -73,9 +81,33 @@ _SIGS = {
     "brtm_op_illumination": [_P, _P],
     "brtm_cg_solve": [_P, _P, _P, C.POINTER(CgParams), _P, _P, C.POINTER(CgStats)],
     "brtm_sinc_table": [_I, _I, _P],
+    # born_solver.h
+    "brtm_linop_born": [_P, C.POINTER(_P)],
+    "brtm_linop_identity": [_P, _L, C.POINTER(_P)],
+    "brtm_linop_diagonal": [_P, C.POINTER(_P)],
+    "brtm_linop_tmute": [_P, _I, _F, _F, _I, _F, _F, _L, _F, _F, C.POINTER(_P)],
+    "brtm_linop_callback": [_P, _L, _L, APPLY_FN, APPLY_FN, _P, C.POINTER(_P)],
+    "brtm_linop_chain": [_I, _P, C.POINTER(_P)],
+    "brtm_linop_column": [_I, _P, C.POINTER(_P)],
+    "brtm_linop_blockdiag": [_I, _P, C.POINTER(_P)],
+    "brtm_linop_set_scale": [_P, _D],
+    "brtm_linop_domain_size": [_P],
+    "brtm_linop_range_size": [_P],
+    "brtm_linop_forward": [_P, _I, _P, _P],
+    "brtm_linop_adjoint": [_P, _I, _P, _P],
+    "brtm_linop_dot_test": [_P, C.c_ulonglong, C.POINTER(_D)],
+    "brtm_linop_destroy": [_P],
+    "brtm_solver_create": [_I, _P, _P, _P, _D, _P, _P, C.POINTER(_P)],
+    "brtm_solver_solve": [_P, C.POINTER(CgParams), _P, _P, C.POINTER(CgStats)],
+    "brtm_solver_return_model": [_P, _P],
+    "brtm_solver_update_model": [_P, _P],
+    "brtm_solver_iterate": [_P],
+    "brtm_solver_residual": [_P],
+    "brtm_solver_destroy": [_P],
 }
 _RES = {"brtm_vec_size": _L, "brtm_op_model_size": _L, "brtm_op_data_size": _L,
-        "brtm_vec_data": _P}
+        "brtm_vec_data": _P, "brtm_linop_domain_size": _L, "brtm_linop_range_size": _L,
+        "brtm_solver_iterate": _P, "brtm_solver_residual": _P}
 _registered = False
 
 
@@ -287,6 +319,12 @@ class RtmZeroOp:
         call("brtm_op_illumination", self.h, v.h)
         return v
 
+    def linop(self) -> "LinOp":
+        """This operator as a composable LinOp (my_operator)."""
+        if getattr(self, "_linop", None) is None:
+            self._linop = BornOp(self)
+        return self._linop
+
     def dot_test(self, verb=False, seed=12345):
         err = (_D * 2)()
         call("brtm_op_dot_test", self.h, C.c_ulonglong(seed), err)
@@ -312,20 +350,221 @@ class SolveResult:
     time_hist: np.ndarray
 
 
-class SimpleSolver:
-    """simple_solver(step, data, op) with solve(niter) (Born/lin_solver.cpp)."""
+def cg_solve(op: RtmZeroOp, data: DeviceVector, model: DeviceVector, niter: int,
+             rel_tol=0.0, verbose=False):
+    """brtm_cg_solve: plain cgstep on the Born operator from m = 0 in one
+    call (the C entry point for callers without the solver objects)."""
+    p = CgParams(int(niter), float(rel_tol), PRECOND_FN(), None, int(verbose), None, 0)
+    rh = np.zeros(niter + 1)
+    th = np.zeros(niter + 1)
+    st = CgStats()
+    call("brtm_cg_solve", op.h, data.h, model.h, C.byref(p), ptr(rh), ptr(th), C.byref(st))
+    k = st.iterations + 1
+    return SolveResult(st.iterations, st.fallbacks, bool(st.converged), bool(st.breakdown),
+                       st.seconds, st.rel_resid, rh[:k], th[:k])
 
-    def __init__(self, step, data: DeviceVector, op: RtmZeroOp, precond=None, verbose=False,
-                 history=0):
-        """history: 0/1 = cgstep's plane search; h >= 2 = flexible CG over the
-        new direction and the last h steps (for preconditioners that change
-        between iterations)."""
+
+# --- operators (my_operator and its kinds) -----------------------------------
+
+
+class LinOp:
+    """my_operator (Born/my_operator.h:7-36) over the library's brtm_linop:
+    forward(add, model, data), adjoint(add, model, data), set_scale(s),
+    dot_test().  Combinations keep references to their parts."""
+
+    def __init__(self, dev, h, parts=()):
+        self.dev = dev
+        self.h = h
+        self._parts = list(parts)
+        dev.adopt(self)
+
+    @property
+    def domain_size(self) -> int:
+        return _lib.load().brtm_linop_domain_size(self.h)
+
+    @property
+    def range_size(self) -> int:
+        return _lib.load().brtm_linop_range_size(self.h)
+
+    def domain_vector(self, a=None):
+        v = DeviceVector(self.dev, self.domain_size)
+        if a is not None:
+            v.upload(a)
+        return v
+
+    def range_vector(self, a=None):
+        v = DeviceVector(self.dev, self.range_size)
+        if a is not None:
+            v.upload(a)
+        return v
+
+    def forward(self, add, model, data, iter=0):
+        call("brtm_linop_forward", self.h, int(bool(add)), model.h, data.h)
+        return True
+
+    def adjoint(self, add, model, data, iter=0):
+        call("brtm_linop_adjoint", self.h, int(bool(add)), model.h, data.h)
+        return True
+
+    def set_scale(self, s):
+        call("brtm_linop_set_scale", self.h, float(s))
+        return self
+
+    def dot_test(self, verb=False, seed=12345):
+        err = (_D * 2)()
+        call("brtm_linop_dot_test", self.h, C.c_ulonglong(seed), err)
+        if verb:
+            print(f"dot test: add=false {err[0]:.3e}, add=true {err[1]:.3e}")
+        return err[0], err[1]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            if not self.dev.alive():
+                self.h = None
+                return
+            call("brtm_linop_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _new(fn, *args):
+    h = _P()
+    call(fn, *args, C.byref(h))
+    return h
+
+
+def as_linop(op) -> LinOp:
+    """An RtmZeroOp as a LinOp (rtm_zero_op is a my_operator)."""
+    if isinstance(op, LinOp):
+        return op
+    if isinstance(op, RtmZeroOp):
+        return op.linop()
+    raise TypeError(f"not an operator: {type(op).__name__}")
+
+
+class BornOp(LinOp):
+    """rtm_zero_op as a composable operator."""
+
+    def __init__(self, op: RtmZeroOp):
+        _ensure()
+        super().__init__(op.dev, _new("brtm_linop_born", op.h), parts=[op])
+
+
+class IOp(LinOp):
+    """i_op (Born/i_op.cpp:4-28): out = scale * in."""
+
+    def __init__(self, dev, n: int):
+        _ensure()
+        super().__init__(dev, _new("brtm_linop_identity", dev.h, int(n)))
+
+
+class WeightOp(LinOp):
+    """Pointwise weight w . in (self-adjoint); keeps w alive."""
+
+    def __init__(self, w: DeviceVector):
+        _ensure()
+        super().__init__(w.dev, _new("brtm_linop_diagonal", w.h), parts=[w])
+
+
+class TMute(LinOp):
+    """tmute (Born/tmute.cpp:4-76) on data [nouter][nrx][nt]: zero the
+    samples before t0 + x_rx / vmute, x_rx = orx + drx * rx."""
+
+    def __init__(self, dev, t0, vmute, nt, ot, dt, nrx, orx, drx, nouter):
+        _ensure()
+        super().__init__(dev, _new("brtm_linop_tmute", dev.h, int(nt), float(ot), float(dt),
+                                   int(nrx), float(orx), float(drx), int(nouter), float(t0),
+                                   float(vmute)))
+
+
+class CallbackOp(LinOp):
+    """A caller-defined operator: forward(add, x, y) computes y (+)= A x and
+    adjoint(add, y, x) computes x (+)= A' y on DeviceVector views."""
+
+    def __init__(self, dev, ndomain, nrange, forward, adjoint):
+        _ensure()
+        nd, nr = int(ndomain), int(nrange)
+
+        def wrap(fn, nin, nout):
+            def cb(user, add, hin, hout):
+                try:
+                    fn(bool(add), DeviceVector.view(dev, hin, nin),
+                       DeviceVector.view(dev, hout, nout))
+                    return 0
+                except Exception as e:  # reported through the C status
+                    print(f"operator callback raised: {e}")
+                    return 1
+            return APPLY_FN(cb)
+
+        self._cbs = (wrap(forward, nd, nr), wrap(adjoint, nr, nd))
+        super().__init__(dev, _new("brtm_linop_callback", dev.h, nd, nr, self._cbs[0],
+                                   self._cbs[1], None))
+
+
+def _combine(fn, ops):
+    ops = [as_linop(o) for o in ops]
+    arr = (_P * len(ops))(*[o.h.value for o in ops])
+    return ops, _new(fn, len(ops), arr)
+
+
+class RowOp(LinOp):
+    """row_op (Born/combo_oper.cpp:176-235): RowOp(A, B) applies A then B."""
+
+    def __init__(self, *ops):
+        _ensure()
+        ops, h = _combine("brtm_linop_chain", ops)
+        super().__init__(ops[0].dev, h, parts=ops)
+
+
+class ColOp(LinOp):
+    """col_op (Born/combo_oper.cpp:1-99): [A m; B m; ...], ranges stacked."""
+
+    def __init__(self, *ops):
+        _ensure()
+        ops, h = _combine("brtm_linop_column", ops)
+        super().__init__(ops[0].dev, h, parts=ops)
+
+
+class DiagOp(LinOp):
+    """diag_op (Born/combo_oper.cpp:103-175): block diagonal."""
+
+    def __init__(self, *ops):
+        _ensure()
+        ops, h = _combine("brtm_linop_blockdiag", ops)
+        super().__init__(ops[0].dev, h, parts=ops)
+
+
+# --- solvers (lin_solver) ----------------------------------------------------
+SOLVER_SIMPLE, SOLVER_REG, SOLVER_PREC = 0, 1, 2
+
+
+class _Solver:
+    """lin_solver (Born/lin_solver.cpp): solve(niter) continues from the
+    current state; return_model() gives the model (the update when m0 was
+    given, as in the reference); update_model(v) adds it to v."""
+
+    def __init__(self, kind, step, data, op, aux, eps, wt, m0, precond, verbose, history):
         assert isinstance(step, cgstep)
-        self.op = op
+        _ensure()
+        self.op = as_linop(op)
+        self.aux = None if aux is None else as_linop(aux)
+        self.wt = None if wt is None else as_linop(wt)
+        self.kind = kind
+        self.dev = self.op.dev
         self.history = int(history)
-        self.data = data
-        self.model = op.model_vector()
         self.verbose = verbose
+        self.data = data
+        self._m0 = m0
+        self.h = _new("brtm_solver_create", kind, self.op.h, data.h,
+                      self.aux.h if self.aux is not None else None, float(eps),
+                      self.wt.h if self.wt is not None else None,
+                      m0.h if m0 is not None else None)
+        self.dev.adopt(self)
         self._precond_cb = None
         self._precond_user = None
         self._precond = precond
@@ -343,9 +582,24 @@ class SimpleSolver:
 
             self._precond_cb = PRECOND_FN(cb)
 
+    @property
+    def model(self) -> DeviceVector:
+        """The current iterate in the solver's domain (a view, updated in place)."""
+        lib = _lib.load()
+        return DeviceVector.view(self.dev, lib.brtm_solver_iterate(self.h),
+                                 (self.aux if self.kind == SOLVER_PREC else self.op).domain_size)
+
+    @property
+    def residual(self) -> DeviceVector:
+        """The stacked residual rr (a view)."""
+        lib = _lib.load()
+        h = lib.brtm_solver_residual(self.h)
+        return DeviceVector.view(self.dev, h, lib.brtm_vec_size(h))
+
     def solve(self, niter, rel_tol=0.0, offset: DeviceVector | None = None) -> SolveResult:
-        """offset: data-space vector e; the judged residual is |rr + e|/|d - e|
-        (with e the synthetic noise: the misfit to the noise-free data)."""
+        """offset: range vector e; the judged residual is |rr + e|/|rr0 + e|
+        (with rr0 = -d and e the synthetic noise: the misfit to the
+        noise-free data)."""
         p = CgParams(int(niter), float(rel_tol),
                      self._precond_cb if self._precond_cb is not None else PRECOND_FN(),
                      self._precond_user, int(self.verbose),
@@ -353,11 +607,63 @@ class SimpleSolver:
         rh = np.zeros(niter + 1)
         th = np.zeros(niter + 1)
         st = CgStats()
-        call("brtm_cg_solve", self.op.h, self.data.h, self.model.h, C.byref(p), ptr(rh), ptr(th),
-             C.byref(st))
+        call("brtm_solver_solve", self.h, C.byref(p), ptr(rh), ptr(th), C.byref(st))
         k = st.iterations + 1
         return SolveResult(st.iterations, st.fallbacks, bool(st.converged), bool(st.breakdown),
                            st.seconds, st.rel_resid, rh[:k], th[:k])
 
-    def return_model(self):
-        return self.model
+    def return_model(self) -> DeviceVector:
+        v = DeviceVector(self.dev, self.op.domain_size)
+        call("brtm_solver_return_model", self.h, v.h)
+        return v
+
+    def update_model(self, model: DeviceVector):
+        call("brtm_solver_update_model", self.h, model.h)
+        return model
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            if not self.dev.alive():
+                self.h = None
+                return
+            call("brtm_solver_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SimpleSolver(_Solver):
+    """simple_solver(step, data, op, wt, m0): min |W(L m - d)|
+    (Born/lin_solver.cpp:7-17)."""
+
+    def __init__(self, step, data: DeviceVector, op, precond=None, verbose=False, history=0,
+                 wt=None, m0=None):
+        """history: 0/1 = cgstep's plane search; h >= 2 = flexible CG over the
+        new direction and the last h steps (for preconditioners that change
+        between iterations)."""
+        super().__init__(SOLVER_SIMPLE, step, data, op, None, 0.0, wt, m0, precond, verbose,
+                         history)
+
+
+class RegSolver(_Solver):
+    """reg_solver(step, data, op, reg, eps, wt, m0): min |W(L m - d)|^2 +
+    |eps R m|^2 (Born/lin_solver.cpp:39-53; sets reg's scale to eps)."""
+
+    def __init__(self, step, data: DeviceVector, op, reg, eps, wt=None, m0=None, precond=None,
+                 verbose=False, history=0):
+        super().__init__(SOLVER_REG, step, data, op, reg, eps, wt, m0, precond, verbose, history)
+
+
+class PrecSolver(_Solver):
+    """prec_solver(step, data, op, prec, eps, wt, m0): m = P p, min
+    |W(L P p - d)|^2 + |eps p|^2 (Born/lin_solver.cpp:21-37); return_model
+    gives P p."""
+
+    def __init__(self, step, data: DeviceVector, op, prec, eps, wt=None, m0=None, precond=None,
+                 verbose=False, history=0):
+        super().__init__(SOLVER_PREC, step, data, op, prec, eps, wt, m0, precond, verbose,
+                         history)

@@ -18,10 +18,13 @@ Parameters (the reference's names and defaults): src_depth=0 rec_depth=0
 bc_a=50 bc_b=0.0005 nbound=bc_a (Born/fd_prop_3d.cpp:37-42), niter=5 (inv).
 aper= and n_gpus= are accepted and have no effect: the reference stores the
 aperture without using it (Born/source_func_3d.cpp:37-63), and one MI355X
-holds the whole survey.  Additional: store=f32|bf16|f16|ckpt (source-wavefield store; ckpt for fwd and
-rtm, bf16 for inv by default), ckpt=, jimg= (imaging cadence, default jt),
+holds the whole survey.  Additional: store=f32|bf16|f16|ckpt (source-wavefield
+store; ckpt for fwd and rtm, bf16 for inv by default), ckpt=, jimg= (imaging
+cadence, default jt),
 rel_tol= (inv: stop at that relative residual), precond=none|diag|learned
-(inv), history= (inv, flexible-CG steps), device= (GPU index).
+(inv; illum_eps= the illumination floor), history= (inv, flexible-CG steps),
+eps= (inv: reg_solver with eps times the identity), tmute= vmute= (inv: the
+tmute data weight, mute before tmute + x/vmute), device= (GPU index).
 
 The time step follows Born/fd_prop_3d.cpp:7-25 (the data interval divided
 into jt equal steps), with the 3-D 8th-order leapfrog limit 0.45 dmin/vmax
@@ -131,14 +134,13 @@ def problem_from_sep(p: sep.Params, store: str) -> SepProblem:
 
 
 def make_operator(dev, sp: SepProblem):
-    from .operator import RtmZeroOp, sinc_table
+    """The Born operator of the survey with its source fields prepared (it
+    installs its own jt-phase sinc tables)."""
+    from .operator import RtmZeroOp
     w = sp.workload
     dev.create_space(w.d, w.bc_a, w.bc_b, w.n)
     dev.set_velocity(w.V)
     dev.set_imaging_scale(w.cimg)
-    tab = sinc_table(w.jt, 8)
-    dev.set_sinc_table(0, tab)
-    dev.set_sinc_table(1, tab)
     op = RtmZeroOp(dev, w, model_box=sp.model_box, mute=sp.mute, nt_data=w.nt)
     op.prepare()
     return op
@@ -155,7 +157,7 @@ def main(argv=None):
         return 2
     prog, p = argv[0], sep.Params(argv[1:])
     from .device import BornDevice
-    from .operator import SimpleSolver, cgstep
+    from .operator import IOp, RegSolver, SimpleSolver, TMute, cgstep
     store = p.get_string("store", "bf16" if prog == "inv" else "ckpt")
     t0 = time.perf_counter()
     sp = problem_from_sep(p, store)
@@ -193,7 +195,7 @@ def main(argv=None):
                 from . import lsrtm
                 mbox = sp.model_box
                 if kind == "diag":
-                    wv = lsrtm.illumination_weight(op, p.get_float("eps", 0.01))
+                    wv = lsrtm.illumination_weight(op, p.get_float("illum_eps", 0.01))
                     precond = lsrtm.diagonal_precond(dev, wv)
                 else:
                     import os
@@ -202,11 +204,25 @@ def main(argv=None):
                     net.load(p.get_string("weights", os.path.join(
                         os.path.dirname(os.path.abspath(__file__)), "weights",
                         "precond_split_e0.01.npy")))
-                    W = lsrtm.illumination_weight(op, p.get_float("eps", 0.01), power=0.5)
+                    W = lsrtm.illumination_weight(op, p.get_float("illum_eps", 0.01), power=0.5)
                     precond = net.precond(mbox[:3], overlap=8, split_weight=W)
                     history = history or 4
-            solver = SimpleSolver(cgstep(), dv, op, precond=precond, verbose=True,
-                                  history=history)
+            wt = None
+            if p.has("tmute"):
+                # the data weight of the reference's commented-out tmute
+                # (Born/RTM_fwd.cpp:50): tmute= t0, vmute= velocity
+                at, arx = sp.data_axes[0], sp.data_axes[1]
+                wt = TMute(dev, p.get_float("tmute"), p.get_float("vmute", 1500.0), nt=at.n,
+                           ot=at.o, dt=at.d, nrx=arx.n, orx=arx.o, drx=arx.d,
+                           nouter=op.nry * len(w.shots))
+            eps = p.get_float("eps", 0.0)
+            if eps > 0:
+                # reg_solver with i_op: damped least squares
+                solver = RegSolver(cgstep(), dv, op, IOp(dev, op.model_size), eps, wt=wt,
+                                   precond=precond, verbose=True, history=history)
+            else:
+                solver = SimpleSolver(cgstep(), dv, op, precond=precond, verbose=True,
+                                      history=history, wt=wt)
             res = solver.solve(p.get_int("niter", 5), rel_tol=p.get_float("rel_tol", 0.0))
             _log(f"{res.iterations} iterations, relative residual {res.rel_resid:.4e}, "
                  f"{res.seconds:.1f} s")

@@ -79,3 +79,19 @@ def test_cli_programs_match_oracle(tmp_path):
     announce(f"  LSRTM model rel err {e:.2e}, residual {np.sqrt(res[-1] / (d @ d)):.3e}")
     assert e < 1e-3
     assert res[-1] < 0.5 * (d @ d)
+
+    announce("cli inv with eps (reg_solver) and tmute")
+    eps = 0.05 * float(np.linalg.norm(d) / np.linalg.norm(m_o))
+    assert rtm_cli.main(["inv", *common, f"data={t}/dout.H", f"image={t}/inv2.H", "niter=3",
+                         f"eps={eps}", "tmute=0.02", "vmute=3000"]) == 0
+    inv2, _ = sep.read(f"{t}/inv2.H")
+    at, arx = daxes[0], daxes[1]
+    start = O.tmute_start(at.n, at.o, at.d, arx.n, arx.o, arx.d, 0.02, 3000.0)
+    mutef = lambda y: O.tmute_apply(y.reshape(-1, arx.n, at.n), start).ravel()  # noqa: E731
+    ident = (lambda m: m, lambda m: m)
+    m_r, _ = O.lin_solve("reg", (lambda m: orc.forward(m).ravel(),
+                                 lambda x: orc.adjoint(x.reshape(Do.shape)).ravel()),
+                         d, 3, aux=ident, eps=eps, W=(mutef, mutef))
+    e = rel(inv2, m_r)
+    announce(f"  regularised, muted LSRTM model rel err {e:.2e}")
+    assert e < 1e-3 and 0 < start.min() and start.max() < at.n

@@ -407,6 +407,67 @@ def test_fcg_full_history_minimises_over_all_directions():
     np.testing.assert_allclose(m, np.linalg.lstsq(A, d, rcond=None)[0], rtol=1e-6, atol=1e-9)
 
 
+# --- solver forms (simple / reg / prec with weight and starting model) ---------
+def solver_case(seed=21, nd=24, nm=10):
+    r = rng(seed)
+    L = r.standard_normal((nd, nm))
+    d = r.standard_normal(nd)
+    W = np.diag(r.uniform(0.5, 2.0, nd))
+    R = np.eye(nm) - np.eye(nm, k=1)          # first difference (nm x nm)
+    P = np.diag(r.uniform(0.2, 3.0, nm))
+    m0 = r.standard_normal(nm)
+    return L, d, W, R, P, m0
+
+
+def test_lin_solve_forms_reach_their_closed_forms():
+    """After as many iterations as unknowns (FP64, exact CG termination up to
+    round-off) each form equals its normal-equation solution; a dropped eps,
+    a transposed weight or the model in place of the update fails here."""
+    L, d, W, R, P, m0 = solver_case()
+    eps = 0.7
+    r = d - L @ m0
+    WL = W @ L
+    m, _ = O.lin_solve("simple", L, d, 30, W=W, m0=m0)
+    np.testing.assert_allclose(m, np.linalg.lstsq(WL, W @ r, rcond=None)[0], rtol=1e-8, atol=1e-10)
+    m, _ = O.lin_solve("reg", L, d, 30, aux=R, eps=eps, W=W, m0=m0)
+    closed = np.linalg.solve(WL.T @ WL + eps ** 2 * R.T @ R, WL.T @ (W @ r))
+    np.testing.assert_allclose(m, closed, rtol=1e-8, atol=1e-10)
+    m, _ = O.lin_solve("prec", L, d, 30, aux=P, eps=eps, W=W)
+    WLP = WL @ P
+    p = np.linalg.solve(WLP.T @ WLP + eps ** 2 * np.eye(10), WLP.T @ (W @ d))
+    np.testing.assert_allclose(m, P @ p, rtol=1e-8, atol=1e-10)
+    # eps = 0 and identity weight reduce to plain cgstep
+    m1, res1 = O.lin_solve("reg", L, d, 6, aux=R, eps=0.0)
+    m2, res2 = O.cgstep_solve(L, d, 6)
+    np.testing.assert_allclose(m1, m2, rtol=1e-10, atol=1e-12)
+
+
+def test_lin_solve_accepts_operator_pairs():
+    L, d, W, R, P, m0 = solver_case(seed=5)
+    a, _ = O.lin_solve("reg", L, d, 4, aux=R, eps=0.3, W=W, m0=m0)
+    pair = lambda M: ((lambda x: M @ x), (lambda y: M.T @ y))  # noqa: E731
+    b, _ = O.lin_solve("reg", pair(L), d, 4, aux=pair(R), eps=0.3, W=pair(W), m0=m0)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_tmute_hand_example():
+    """t0 = 0.1 s, vmute = 2000 m/s, receivers at x = 100 + 50 i m, 4 ms
+    samples from ot = 0: mute times 0.15, 0.175, 0.2 s -> samples 37, 43, 50
+    (truncated in FP32: 37.5 -> 37, 43.75 -> 43, 0.2/0.004 -> 50); a mute
+    time before ot starts at 0, one past the trace keeps nothing."""
+    s = O.tmute_start(100, 0.0, 0.004, 3, 100.0, 50.0, 0.1, 2000.0)
+    assert list(s) == [37, 43, 50]
+    assert list(O.tmute_start(10, 0.5, 0.004, 2, 0.0, 10.0, 0.1, 2000.0)) == [0, 0]
+    assert list(O.tmute_start(10, 0.0, 0.004, 1, 0.0, 10.0, 5.0, 2000.0)) == [10]
+    x = np.arange(2 * 3 * 100, dtype=np.float64).reshape(2, 3, 100)
+    y = O.tmute_apply(x, s)
+    assert (y[:, 0, :37] == 0).all() and (y[:, 0, 37:] == x[:, 0, 37:]).all()
+    # a 0/1 mask: self-adjoint and idempotent
+    z = rng(2).standard_normal(x.shape)
+    assert np.isclose((y * z).sum(), (x * O.tmute_apply(z, s)).sum())
+    np.testing.assert_array_equal(O.tmute_apply(y, s), y)
+
+
 # --- geometry ------------------------------------------------------------------
 def test_source_geometry_hand_example():
     locs, tr = O.source_geometry(10, 12, 5.0, 4.0, 6.0, np.array([1.0, 2.0]), 8)
