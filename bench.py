@@ -11,15 +11,16 @@ value = RTM Gpts/s = 2 * n1*n2*n3 * nt / time per shot (the two wavefields
 an RTM must advance; the checkpoint recompute is overhead, not credited).
 
 roofline: the dominant kernel is the fused VL stencil step (k_step_x4r at
-512^3: four cells per lane on the six-plane halo ring, no imaging; it runs for the source propagation
-and the recompute).  Its launches inside the timed region are bracketed by
+512^3: four cells per lane on the six-plane halo ring, no imaging; it runs
+for the source propagation and the recompute).  Its launches inside the timed region are bracketed by
 HIP events on the library stream; achieved = 16 B per cell (read p_n,
 p_{n-1}, V; write p_{n+1}) * n1*n2*n3 / mean launch duration.  traffic is
 the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 (scaled by a 1 GiB copy's counters) for the default configuration.
 
 cpu_baseline: the FP32 OpenMP CPU restatement (oracle/liborc32.so) of the
-same stencil step at 512^3, timed on the host cores for ~10 s.
+same stencil step at 512^3, timed on the host cores: the median of three
+samples of ~3.3 s each.
 """
 from __future__ import annotations
 
@@ -77,11 +78,25 @@ def rtm_shot(dev, w):
     dev.rtm_adjoint(w.jt, w.nt, w.jimg)
 
 
-def cpu_baseline(w, seconds=10.0):
-    """FP32 OpenMP restatement of the VL step at the workload's size."""
-    from oracle import born_oracle as O
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, seconds=10.0, runs=3):
+    """FP32 OpenMP restatement of the VL step at the workload's size: the
+    median per-step rate of `runs` timed samples of seconds/runs each."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from oracle import born_oracle as O  # loads libgomp after the settings above
     P = O.Problem(w.n, w.d, w.V, w.bc_a, w.bc_b, prec=32)
     rng = np.random.default_rng(0)
     cur = np.zeros(w.ncell, np.float32)
@@ -90,17 +105,25 @@ def cpu_baseline(w, seconds=10.0):
     v[4:-4, 4:-4, 4:-4] = rng.standard_normal((w.n[2] - 8, w.n[1] - 8, w.n[0] - 8))
     import ctypes as C
     P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)  # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)
-        steps += 1
-        el = time.perf_counter() - t0
-        if (el >= seconds and steps >= 3) or steps >= 1000:
-            break
-    gpts = w.ncell * steps / el / 1e9
+    rates, total_steps, total_s = [], 0, 0.0
+    for _ in range(runs):
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)
+            steps += 1
+            el = time.perf_counter() - t0
+            if (el >= seconds / runs and steps >= 2) or steps >= 1000:
+                break
+        rates.append(w.ncell * steps / el / 1e9)
+        total_steps += steps
+        total_s += el
+    gpts = float(np.median(rates))
     return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} FP32 VL stencil steps (fused sponge) of the {w.n[0]}x{w.n[1]}x"
-                      f"{w.n[2]} grid on {threads} OpenMP threads, {el:.1f} s; "
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"median of {runs} samples ({total_steps} FP32 VL stencil steps with "
+                      f"fused sponge in all, {total_s:.1f} s) of the {w.n[0]}x{w.n[1]}x"
+                      f"{w.n[2]} grid on {threads} OpenMP threads (OMP_PROC_BIND="
+                      f"{os.environ['OMP_PROC_BIND']}, OMP_PLACES={os.environ['OMP_PLACES']}); "
                       "per-step rate, not the full 4000-step shot"}
 
 
