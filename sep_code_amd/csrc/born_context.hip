@@ -81,7 +81,9 @@ static hipError_t h2d(brtm_ctx *c, void *dst, const void *src, size_t bytes) {
 }
 
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
-         const void *snap, const float *sscale, float *img, const float *m) {
+         const void *snap, const float *sscale, float *img, const float *m, int rec, int it,
+         int jt, const float *rscale, bool *fused) {
+  if (fused) *fused = false;
   StepParams P{};
   P.n1 = c->n1;
   P.n2 = c->n2;
@@ -139,10 +141,22 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
+  // receivers inside the step: four-cell ring kernel, one receiver plane
+  const bool in_step = rec != REC_NONE && fused && c->rec_fusion && c->rec_plane &&
+                       c->nrec > 0 && xl == 4 && ring && form == FORM_VL;
+  if (in_step) {
+    P.rz = c->rec_z;
+    P.nrec = c->nrec;
+    P.rid = it / jt;
+    P.rmap = c->rmap.as<int>();
+    P.rtr = c->traces.as<float>();
+    P.rscale = rec == REC_EXTRACT ? rscale : nullptr;
+    std::memcpy(P.rsinc, &c->sinc[1][8 * (it % jt)], sizeof(P.rsinc));
+  }
   if (xl == 1 && form == FORM_VL && ring)
     BRTM_HIP(launch_step_ring(act, fmt, P, ty, c->stream));
   else if (xl == 4)
-    BRTM_HIP(launch_step_x4(act, fmt, P, ty, ring, c->stream));
+    BRTM_HIP(launch_step_x4(act, fmt, P, ty, ring, c->stream, in_step ? rec : REC_NONE));
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
@@ -151,6 +165,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
     c->ktime_used += 2;
   }
+  if (fused) *fused = in_step;
   return BRTM_OK;
 }
 
@@ -320,6 +335,7 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
     BRTM_HIP(hipMemsetAsync(so, 0, fbytes, c->stream));
   }
   for (int it = 0; it < nt; ++it) {
+    bool fused = false;
     if (live) {
       BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, sc, so, so, nullptr, nullptr, nullptr,
                     nullptr));
@@ -338,12 +354,13 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
         snap = S.data.as<char>() + fmt_bytes(f) * (size_t)c->ncell * k;
         if (f != FMT_F32) ss = S.scale.as<float>() + (size_t)c->n3 * k;
       }
-      BRTM_TRY(step(c, qform, ACT_BORN, f, qc, qo, qo, snap, ss, nullptr, mdl));
+      BRTM_TRY(step(c, qform, ACT_BORN, f, qc, qo, qo, snap, ss, nullptr, mdl, REC_EXTRACT, it,
+                    jt, rscale, &fused));
     } else {
       BRTM_TRY(step(c, qform, ACT_NONE, FMT_NONE, qc, qo, qo, nullptr, nullptr, nullptr,
-                    nullptr));
+                    nullptr, REC_EXTRACT, it, jt, rscale, &fused));
     }
-    BRTM_TRY(extract_data(c, it, jt, qo, rscale));
+    if (!fused) BRTM_TRY(extract_data(c, it, jt, qo, rscale));
     std::swap(qc, qo);
     if (live) std::swap(sc, so);
   }
@@ -371,12 +388,15 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
         const int k = it / jimg;
         const void *snap = S.data.as<char>() + fmt_bytes(sfmt) * (size_t)c->ncell * k;
         const float *ss = sfmt == FMT_F32 ? nullptr : S.scale.as<float>() + (size_t)c->n3 * k;
-        BRTM_TRY(step(c, FORM_VL, ACT_IMG, sfmt, lc, lo, lo, snap, ss, img, nullptr));
-        BRTM_TRY(inject_data(c, it, jt, lo, snap, sfmt, ss, img));
+        bool fused = false;
+        BRTM_TRY(step(c, FORM_VL, ACT_IMG, sfmt, lc, lo, lo, snap, ss, img, nullptr, REC_INJECT,
+                      it, jt, nullptr, &fused));
+        if (!fused) BRTM_TRY(inject_data(c, it, jt, lo, snap, sfmt, ss, img));
       } else {
+        bool fused = false;
         BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr,
-                      nullptr));
-        BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
+                      nullptr, REC_INJECT, it, jt, nullptr, &fused));
+        if (!fused) BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
       }
       std::swap(lc, lo);
     }
@@ -405,12 +425,15 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
     for (int it = k1 - 1; it >= k0; --it) {
       if (it % jimg == 0) {
         const float *snap = slot(it - k0 + 2);
-        BRTM_TRY(step(c, FORM_VL, ACT_IMG, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr));
-        BRTM_TRY(inject_data(c, it, jt, lo, snap, FMT_F32, nullptr, img));
+        bool fused = false;
+        BRTM_TRY(step(c, FORM_VL, ACT_IMG, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr,
+                      REC_INJECT, it, jt, nullptr, &fused));
+        if (!fused) BRTM_TRY(inject_data(c, it, jt, lo, snap, FMT_F32, nullptr, img));
       } else {
+        bool fused = false;
         BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, lc, lo, lo, nullptr, nullptr, nullptr,
-                      nullptr));
-        BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
+                      nullptr, REC_INJECT, it, jt, nullptr, &fused));
+        if (!fused) BRTM_TRY(inject_data(c, it, jt, lo, nullptr, FMT_NONE, nullptr, nullptr));
       }
       std::swap(lc, lo);
     }
@@ -467,7 +490,7 @@ int brtm_destroy(brtm_ctx *c) {
     s.scale.release();
   }
   for (DevBuf *b : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
-                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap,
                     &c->traces})
     b->release();
   for (hipEvent_t e : c->ktime_ev) (void)hipEventDestroy(e);
@@ -624,6 +647,27 @@ int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, co
     if (!in_update_set(c, l)) l = -1;
   BRTM_TRY(c->rlocs.alloc(sizeof(int) * nrec));
   BRTM_HIP(h2d(c, c->rlocs.ptr, c->h_rlocs.data(), sizeof(int) * nrec));
+  // One plane, one receiver per cell: the step kernel can take them.
+  {
+    const long n12 = (long)c->n1 * c->n2;
+    std::vector<int> map(n12, -1);
+    int z = -1;
+    bool ok = true;
+    for (int j = 0; j < nrec && ok; ++j) {
+      const long l = c->h_rlocs[j];
+      if (l < 0) continue;
+      const int lz = (int)(l / n12);
+      if (z < 0) z = lz;
+      if (lz != z || map[l % n12] >= 0) ok = false;
+      else map[l % n12] = j;
+    }
+    c->rec_plane = ok && z >= 0;
+    c->rec_z = c->rec_plane ? z : -1;
+    if (c->rec_plane) {
+      BRTM_TRY(c->rmap.alloc(sizeof(int) * (size_t)n12));
+      BRTM_HIP(h2d(c, c->rmap.ptr, map.data(), sizeof(int) * (size_t)n12));
+    }
+  }
   const size_t tb = sizeof(float) * (size_t)nrec * ntr;
   BRTM_TRY(c->traces.alloc(tb));
   if (!vals) {
@@ -807,6 +851,12 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   return BRTM_OK;
 }
 
+int brtm_set_receiver_fusion(brtm_ctx *c, int on) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  c->rec_fusion = on != 0;
+  return BRTM_OK;
+}
+
 int brtm_set_halo_ring(brtm_ctx *c, int on) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
   BRTM_REQUIRE(on >= -1 && on <= 1, BRTM_EINVAL, "halo ring: -1 (automatic), 0 or 1");
@@ -882,7 +932,7 @@ int brtm_device_bytes(brtm_ctx *c, double *bytes) {
   BRTM_REQUIRE(c && bytes, BRTM_EINVAL, "NULL argument");
   double b = 0;
   for (DevBuf *x : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
-                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap,
                     &c->traces, &c->mvw, &c->rscale})
     b += (double)x->bytes;
   for (auto &s : c->stores) b += (double)s.data.bytes + (double)s.scale.bytes;

@@ -73,6 +73,7 @@ struct brtm_ctx {
   int ty_tile = 0, target_blocks = 2048;  // ty_tile 0: chosen per step variant
   int x_per_lane = 0;  // VL steps: 0 automatic, 1, 2 (n1 even) or 4 (n1 % 4 == 0)
   int halo_ring = -1;  // six-plane LDS ring: -1 automatic (on for 4 cells per lane), 0, 1
+  bool rec_fusion = true;  // receivers inside the four-cell ring step where they allow it
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)
@@ -90,6 +91,12 @@ struct brtm_ctx {
   int nrx = 0, nry = 0, nrec = 0, ntr_d = 0;
   brtm::DevBuf rlocs, traces;  // traces [ntr_d][nrec]
   std::vector<int> h_rlocs;
+  // When the active receivers share one plane (rec_z) with one receiver per
+  // cell, rmap [n2][n1] holds each cell's receiver index (-1: none) and the
+  // step kernel injects / extracts them itself.
+  bool rec_plane = false;
+  int rec_z = -1;
+  brtm::DevBuf rmap;
 
   std::vector<brtm::Store> stores;
   int shot = 0;
@@ -115,8 +122,13 @@ int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out);
 // acc += sum over the current shot's stored snapshots of s^2 (compute grid).
 int illumination(brtm_ctx *c, float *acc);
 Store &cur_store(brtm_ctx *c);
+// rec (RecMode): receivers handled by the step at time index it (jt steps
+// per sample, rscale for extraction) when *fused comes back true; the caller
+// runs inject_data / extract_data otherwise.
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
-         const void *snap, const float *sscale, float *img_or_null, const float *m_or_null);
+         const void *snap, const float *sscale, float *img_or_null, const float *m_or_null,
+         int rec = REC_NONE, int it = 0, int jt = 1, const float *rscale = nullptr,
+         bool *fused = nullptr);
 
 }  // namespace brtm
 

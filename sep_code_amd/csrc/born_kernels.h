@@ -14,6 +14,11 @@ enum StepForm { FORM_VL = 0, FORM_LV = 1 };
 //   ACT_IMG  img += cimg . s . p_new        (zero-lag imaging, VL only)
 //   ACT_BORN p_new += cimg . s . m          (Born secondary source)
 enum StepAct { ACT_NONE = 0, ACT_IMG = 1, ACT_BORN = 2 };
+// Receivers handled inside the step (the four-cell ring kernel):
+//   REC_INJECT   pn += sum_k rsinc[k] rtr(r, rid+k) before imaging and the store
+//   REC_EXTRACT  rtr(r, rid+k) += rsinc[k] . pn . (rscale[r] or 1)
+// for the cells of plane rz whose rmap entry is a receiver index r >= 0.
+enum RecMode { REC_NONE = 0, REC_INJECT = 1, REC_EXTRACT = 2 };
 
 constexpr int kWMax = 256;  // longest sponge table the kernels accept
 
@@ -36,6 +41,13 @@ struct StepParams {
   const float *sscale;  // [n3] slice scales (16-bit formats), else null
   const float *m;       // LV: Born model
   float cimg;
+  // receivers (RecMode): rmap [n2][n1] receiver index or -1 on plane rz;
+  // traces rtr[t * nrec + r] (time-major), time index rid, taps rsinc
+  int rz, nrec, rid;
+  const int *rmap;
+  float *rtr;
+  const float *rscale;
+  float rsinc[8];
 };
 
 hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
@@ -44,9 +56,10 @@ hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStre
 // The VL step with two consecutive x per lane (128-wide tiles, n1 even).
 hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with four consecutive x per lane (256-wide tiles, n1 % 4 == 0),
-// on the two-plane buffer or (ring) the six-plane halo ring.
+// on the two-plane buffer or (ring) the six-plane halo ring; rec (RecMode,
+// ring only) handles the receivers inside the step.
 hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring,
-                          hipStream_t st);
+                          hipStream_t st, int rec = REC_NONE);
 // Tile / chunk geometry chosen for a grid (fills ntx..zc of P); xw is the
 // tile width in x (64, 128 or 256 for the 1-, 2- and 4-cell kernels).
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw = 64);

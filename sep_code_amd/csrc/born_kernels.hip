@@ -890,10 +890,12 @@ k_step_x4(const StepParams P) {
 // 149 KB of LDS, which the one workgroup per CU that the registers allow
 // can have.
 // ---------------------------------------------------------------------------
-template <int TY, int RPT, int FMT, int ACT>
+template <int TY, int RPT, int FMT, int ACT, int REC = REC_NONE>
 __global__ void __launch_bounds__(64 * (TY / RPT))
 k_step_x4r(const StepParams P) {
   static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  static_assert(!(REC == REC_INJECT && ACT == ACT_BORN) && !(REC == REC_EXTRACT && ACT == ACT_IMG),
+                "injection goes with the adjoint, extraction with Born modelling");
   constexpr int NWV = TY / RPT;
   constexpr int HY = 8 / NWV;
   static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
@@ -1065,6 +1067,31 @@ k_step_x4r(const StepParams P) {
           float v = w * fmaf(cV[r][j], acc[j], 2.f * C[j]) - (w * w) * cpo[r][j];
           if constexpr (ACT == ACT_BORN) v = fmaf(P.cimg * cS[r][j], cX[r][j], v);
           pn[j] = v;
+        }
+        if constexpr (REC != REC_NONE) {
+          // the receiver plane: the same arithmetic as k_inject / k_extract
+          // for the (at most one) receiver of each cell
+          if (z == P.rz) {
+            const int4 rr = *reinterpret_cast<const int4 *>(P.rmap + y * n1 + x);
+            const int ri[4] = {rr.x, rr.y, rr.z, rr.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              if (ri[j] < 0) continue;
+              if constexpr (REC == REC_INJECT) {
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                  if (P.rsinc[k] != 0.f)
+                    acc = fmaf(P.rsinc[k], P.rtr[(long)(P.rid + k) * P.nrec + ri[j]], acc);
+                pn[j] += acc;
+              } else {
+                const float v = P.rscale ? pn[j] * P.rscale[ri[j]] : pn[j];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                  if (P.rsinc[k] != 0.f) P.rtr[(long)(P.rid + k) * P.nrec + ri[j]] += v * P.rsinc[k];
+              }
+            }
+          }
         }
         __builtin_nontemporal_store(pn, reinterpret_cast<f32x4 *>(P.pn + i));
         if constexpr (ACT == ACT_IMG) {
@@ -1444,35 +1471,50 @@ hipError_t launch_ring_t(int act, int fmt, const StepParams &P, hipStream_t st) 
 }
 
 template <bool RING, int TY, int RPT, int FMT, int ACT>
-void launch_x4_kernel(const StepParams &P, hipStream_t st) {
+hipError_t launch_x4_kernel(const StepParams &P, hipStream_t st, int rec) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.nty * P.ntz);
-  if constexpr (RING)
-    k_step_x4r<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
-  else
+  if constexpr (RING) {
+    if (rec == REC_NONE) {
+      k_step_x4r<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
+    } else if (rec == REC_INJECT) {
+      if constexpr (ACT != ACT_BORN)
+        k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT><<<grid, block, 0, st>>>(P);
+      else
+        return hipErrorInvalidValue;
+    } else if (rec == REC_EXTRACT) {
+      if constexpr (ACT != ACT_IMG)
+        k_step_x4r<TY, RPT, FMT, ACT, REC_EXTRACT><<<grid, block, 0, st>>>(P);
+      else
+        return hipErrorInvalidValue;
+    } else {
+      return hipErrorInvalidValue;
+    }
+  } else {
+    if (rec != REC_NONE) return hipErrorInvalidValue;
     k_step_x4<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
-}
-
-template <bool RING, int TY, int RPT, int ACT>
-hipError_t launch_x4_a(int fmt, const StepParams &P, hipStream_t st) {
-  switch (fmt) {
-    case FMT_F32: launch_x4_kernel<RING, TY, RPT, FMT_F32, ACT>(P, st); break;
-    case FMT_BF16: launch_x4_kernel<RING, TY, RPT, FMT_BF16, ACT>(P, st); break;
-    case FMT_F16: launch_x4_kernel<RING, TY, RPT, FMT_F16, ACT>(P, st); break;
-    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+template <bool RING, int TY, int RPT, int ACT>
+hipError_t launch_x4_a(int fmt, const StepParams &P, hipStream_t st, int rec) {
+  switch (fmt) {
+    case FMT_F32: return launch_x4_kernel<RING, TY, RPT, FMT_F32, ACT>(P, st, rec);
+    case FMT_BF16: return launch_x4_kernel<RING, TY, RPT, FMT_BF16, ACT>(P, st, rec);
+    case FMT_F16: return launch_x4_kernel<RING, TY, RPT, FMT_F16, ACT>(P, st, rec);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <bool RING, int TY, int RPT>
-hipError_t launch_x4_t(int act, int fmt, const StepParams &P, hipStream_t st) {
+hipError_t launch_x4_t(int act, int fmt, const StepParams &P, hipStream_t st, int rec) {
   if (act == ACT_NONE) {
     if (fmt != FMT_NONE) return hipErrorInvalidValue;
-    launch_x4_kernel<RING, TY, RPT, FMT_NONE, ACT_NONE>(P, st);
-    return hipGetLastError();
+    return launch_x4_kernel<RING, TY, RPT, FMT_NONE, ACT_NONE>(P, st, rec);
   }
-  if (act == ACT_BORN) return launch_x4_a<RING, TY, RPT, ACT_BORN>(fmt, P, st);
-  if (act == ACT_IMG) return launch_x4_a<RING, TY, RPT, ACT_IMG>(fmt, P, st);
+  if (act == ACT_BORN) return launch_x4_a<RING, TY, RPT, ACT_BORN>(fmt, P, st, rec);
+  if (act == ACT_IMG) return launch_x4_a<RING, TY, RPT, ACT_IMG>(fmt, P, st, rec);
   return hipErrorInvalidValue;
 }
 
@@ -1521,16 +1563,18 @@ hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStre
 }
 
 hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring,
-                          hipStream_t st) {
+                          hipStream_t st, int rec) {
   if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 12 || (P.n1 & 3)) return hipErrorInvalidValue;
+  if (rec != REC_NONE && (!P.rmap || !P.rtr || P.nrec <= 0 || P.rz < 4 || P.rz >= P.n3 - 4))
+    return hipErrorInvalidValue;
   // 8-row tiles with one row per thread, or 16-row tiles with two (ty_tile
   // 16 or 32); four cells per lane leave no registers for taller tiles.
   if (ring) {
-    if (ty_tile == 8) return launch_x4_t<true, 8, 1>(act, fmt, P, st);
-    return launch_x4_t<true, 16, 2>(act, fmt, P, st);
+    if (ty_tile == 8) return launch_x4_t<true, 8, 1>(act, fmt, P, st, rec);
+    return launch_x4_t<true, 16, 2>(act, fmt, P, st, rec);
   }
-  if (ty_tile == 8) return launch_x4_t<false, 8, 1>(act, fmt, P, st);
-  return launch_x4_t<false, 16, 2>(act, fmt, P, st);
+  if (ty_tile == 8) return launch_x4_t<false, 8, 1>(act, fmt, P, st, rec);
+  return launch_x4_t<false, 16, 2>(act, fmt, P, st, rec);
 }
 
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw) {

@@ -189,6 +189,11 @@ class BornDevice:
         1 on, 0 off, -1 automatic (the default: on for four cells per lane)."""
         call("brtm_set_halo_ring", self.h, -1 if on is None or on == -1 else int(bool(on)))
 
+    def set_receiver_fusion(self, on):
+        """Receivers injected / extracted inside the four-cell ring step when
+        they share one plane (1, the default) or by separate kernels (0)."""
+        call("brtm_set_receiver_fusion", self.h, int(bool(on)))
+
     def set_x_per_lane(self, cells):
         """1: 64-wide tiles; 2: 128-wide tiles with 8-byte accesses (VL, n1
         even); 4: 256-wide tiles with 16-byte accesses (VL, n1 % 4 == 0)."""

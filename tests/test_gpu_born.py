@@ -208,6 +208,76 @@ def test_actions_variants(dev, store, v):
         assert rel(out[v][1], P.rtm_adjoint(Do, snaps)) < 1e-4
 
 
+@pytest.mark.parametrize("jt,store", [(1, "f32"), (2, "f32"), (1, "bf16")])
+def test_receiver_fusion(dev, jt, store):
+    """Receivers injected / extracted inside the four-cell ring step: traces
+    bit-identical to the separate extraction kernel, images to FP32 rounding
+    of the separate injection kernel, both against the oracle."""
+    announce(f"receiver fusion jt={jt} store={store}")
+    P, r = make_shot(dev, n=(52, 44, 40), jt=jt, jimg=2)
+    m = random_field(r, P.n)
+    snaps, _ = P.source_prop()
+    Do = P.born_forward(m.ravel(), snaps)
+    out = {}
+    set_variant(dev, (4, 1))
+    try:
+        for fuse in (0, 1):
+            dev.set_receiver_fusion(fuse)
+            dev.set_receivers(P.rlocs, None, ntr=P.ntr_d, nx=len(P.rlocs), ny=1)
+            dev.source_prop(jt, P.nt, 2, store=store)
+            dev.set_model(m)
+            dev.rtm_forward(jt, P.nt, 2)
+            D = dev.get_receivers()
+            dev.set_receivers(P.rlocs, Do)
+            dev.zero_image()
+            dev.rtm_adjoint(jt, P.nt, 2)
+            out[fuse] = (D, dev.get_image())
+    finally:
+        dev.set_receiver_fusion(1)
+        set_variant(dev, (0, -1))
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    assert rel(out[1][1], out[0][1]) < 1e-6
+    if store == "f32":
+        assert rel(out[1][0], Do) < 1e-5
+        assert rel(out[1][1], P.rtm_adjoint(Do, snaps)) < 1e-4
+
+
+@pytest.mark.parametrize("layout", ["duplicate", "two_planes"])
+def test_receivers_off_one_plane_fall_back(dev, layout):
+    """Receivers sharing a cell, or on two depths, cannot go through the
+    per-plane receiver map: the separate kernels run, and results still match
+    the oracle."""
+    announce(f"receiver fallback {layout}")
+    P, r = make_shot(dev, n=(52, 44, 40), jimg=2)
+    n1, n2 = P.n[0], P.n[1]
+    rl = P.rlocs.copy()
+    act = np.flatnonzero(rl >= 0)
+    if layout == "duplicate":
+        rl[act[1]] = rl[act[0]]
+    else:
+        rl[act[::2]] += n1 * n2 * 3
+    Q = O.Problem(P.n, (10.0, 10.0, 10.0), P.V.reshape(P.n[::-1]), 8.0, 0.02, src_locs=P.slocs,
+                  src_tr=P.strc, rec_locs=rl, ntr_d=P.ntr_d, jt=1, nt=P.nt, jimg=2, cimg=1e-3)
+    m = random_field(r, P.n)
+    snaps, _ = Q.source_prop()
+    Do = Q.born_forward(m.ravel(), snaps)
+    set_variant(dev, (4, 1))
+    try:
+        dev.set_receivers(rl, None, ntr=P.ntr_d, nx=len(rl), ny=1)
+        dev.source_prop(1, P.nt, 2, store="f32")
+        dev.set_model(m)
+        dev.rtm_forward(1, P.nt, 2)
+        D = dev.get_receivers()
+        dev.set_receivers(rl, Do)
+        dev.zero_image()
+        dev.rtm_adjoint(1, P.nt, 2)
+        img = dev.get_image()
+    finally:
+        set_variant(dev, (0, -1))
+    assert rel(D, Do) < 1e-5
+    assert rel(img, Q.rtm_adjoint(Do, snaps)) < 1e-4
+
+
 @pytest.mark.parametrize("store,bits", [("bf16", 8), ("f16", 11)])
 def test_snapshot_codec_roundtrip(dev, store, bits):
     """Per-z-slice scaled 16-bit snapshots: every value within half an ulp of
