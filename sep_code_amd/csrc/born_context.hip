@@ -258,30 +258,58 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     S.count = (nt + ckpt - 1) / ckpt;
     BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
   }
-  float *cur = c->fb.as<float>(), *old = c->fa.as<float>();
-  BRTM_HIP(hipMemsetAsync(cur, 0, fbytes, c->stream));
-  BRTM_HIP(hipMemsetAsync(old, 0, fbytes, c->stream));
+  float *const wx = c->fb.as<float>(), *const wy = c->fa.as<float>();
+  float *cur = wx, *old = wy;  // s^it, s^{it-1}
+  // Checkpoint pair K = (s^{K ckpt}, s^{K ckpt - 1}).  With ckpt >= 2 the
+  // two steps before each checkpoint write straight into its pair, and the
+  // steps that read a pair write out of place into a work buffer.
+  const bool ck_direct = kind == BRTM_STORE_CKPT && ckpt >= 2;
+  auto ck_cur = [&](int K) {
+    return reinterpret_cast<float *>(S.data.as<char>() + 2 * fbytes * (size_t)K);
+  };
+  if (ck_direct) {
+    // the steps write only the update set: the pairs' halos must be zero
+    // like the work buffers', and pair 0 is (s^0, s^-1) = 0
+    BRTM_HIP(hipMemsetAsync(ck_cur(0), 0, 2 * fbytes, c->stream));
+    BRTM_HIP(launch_zero_halo(ck_cur(1), c->ncell, 2 * (S.count - 1), c->n1, c->n2, c->n3,
+                              c->stream));
+    cur = ck_cur(0);
+    old = ck_cur(0) + c->ncell;
+  } else {
+    BRTM_HIP(hipMemsetAsync(cur, 0, fbytes, c->stream));
+    BRTM_HIP(hipMemsetAsync(old, 0, fbytes, c->stream));
+  }
   const int n12 = c->n1 * c->n2;
   for (int it = 0; it < nt; ++it) {
-    if (kind == BRTM_STORE_CKPT && it % ckpt == 0) {
+    if (kind == BRTM_STORE_CKPT && !ck_direct && it % ckpt == 0) {
       char *base = S.data.as<char>() + 2 * fbytes * (size_t)(it / ckpt);
       BRTM_HIP(hipMemcpyAsync(base, cur, fbytes, hipMemcpyDeviceToDevice, c->stream));
       BRTM_HIP(hipMemcpyAsync(base + fbytes, old, fbytes, hipMemcpyDeviceToDevice, c->stream));
     }
-    BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, old, nullptr, nullptr, nullptr,
+    float *out = old;  // in place over s^{it-1}
+    if (ck_direct) {
+      if ((it + 1) % ckpt == 0 && (it + 1) / ckpt < S.count)
+        out = ck_cur((it + 1) / ckpt);                 // s^{it+1} opens checkpoint K
+      else if ((it + 2) % ckpt == 0 && (it + 2) / ckpt < S.count)
+        out = ck_cur((it + 2) / ckpt) + c->ncell;      // s^{it+1} = s^{K ckpt - 1}
+      else if (old != wx && old != wy)
+        out = cur == wx ? wy : wx;                     // s^{it-1} is a checkpoint: keep it
+    }
+    BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, out, nullptr, nullptr, nullptr,
                   nullptr));
-    BRTM_TRY(inject_source(c, it, jt, old));
+    BRTM_TRY(inject_source(c, it, jt, out));
     if (fmt != FMT_NONE && it % jimg == 0) {
       const int k = it / jimg;
       if (fmt == FMT_F32) {
-        BRTM_HIP(hipMemcpyAsync(S.data.as<char>() + fbytes * (size_t)k, old, fbytes,
+        BRTM_HIP(hipMemcpyAsync(S.data.as<char>() + fbytes * (size_t)k, out, fbytes,
                                 hipMemcpyDeviceToDevice, c->stream));
       } else {
-        BRTM_HIP(launch_encode(fmt, old, S.data.as<char>() + 2 * (size_t)c->ncell * k,
+        BRTM_HIP(launch_encode(fmt, out, S.data.as<char>() + 2 * (size_t)c->ncell * k,
                                S.scale.as<float>() + (size_t)c->n3 * k, n12, c->n3, c->stream));
       }
     }
-    std::swap(cur, old);
+    old = cur;
+    cur = out;
   }
   if (p0) BRTM_HIP(hipMemcpyAsync(p0, old, fbytes, hipMemcpyDeviceToHost, c->stream));
   if (p1) BRTM_HIP(hipMemcpyAsync(p1, cur, fbytes, hipMemcpyDeviceToHost, c->stream));
@@ -403,28 +431,33 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
     return BRTM_OK;
   }
   // Checkpoints: recompute each segment of the source field out of place
-  // into seg slots 0..ckpt+1 (slot j holds s^{k0-1+j}), then step the
-  // receiver field back through the segment.
+  // into seg slots 0..ckpt-1 (slot j holds s^{k0+1+j}), starting from the
+  // checkpoint pair itself (no copy), then step the receiver field back
+  // through the segment.
   const int ck = S.ckpt;
-  if (c->seg_slots < ck + 2) {
-    BRTM_TRY(c->seg.alloc(fbytes * (size_t)(ck + 2)));
-    c->seg_slots = ck + 2;
+  if (c->seg_slots < ck) {
+    // the recompute writes only the update set: keep the slots' halos zero
+    BRTM_TRY(c->seg.alloc(fbytes * (size_t)ck));
+    BRTM_HIP(launch_zero_halo(c->seg.as<float>(), c->ncell, ck, c->n1, c->n2, c->n3, c->stream));
+    c->seg_slots = ck;
   }
   auto slot = [&](int j) { return c->seg.as<float>() + (size_t)c->ncell * j; };
   for (int cidx = S.count - 1; cidx >= 0; --cidx) {
     const int k0 = cidx * ck, k1 = std::min(k0 + ck, nt);
-    const char *base = S.data.as<char>() + 2 * fbytes * (size_t)cidx;
-    BRTM_HIP(hipMemcpyAsync(slot(1), base, fbytes, hipMemcpyDeviceToDevice, c->stream));
-    BRTM_HIP(hipMemcpyAsync(slot(0), base + fbytes, fbytes, hipMemcpyDeviceToDevice, c->stream));
+    const float *ck_p = reinterpret_cast<const float *>(S.data.as<char>() +
+                                                        2 * fbytes * (size_t)cidx);  // s^{k0}
+    const float *ck_po = ck_p + c->ncell;                                            // s^{k0-1}
     for (int it = k0; it < k1; ++it) {
       const int j = it - k0;
-      BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, slot(j + 1), slot(j), slot(j + 2), nullptr,
-                    nullptr, nullptr, nullptr));
-      BRTM_TRY(inject_source(c, it, jt, slot(j + 2)));
+      const float *p = j == 0 ? ck_p : slot(j - 1);
+      const float *po = j == 0 ? ck_po : (j == 1 ? ck_p : slot(j - 2));
+      BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, p, po, slot(j), nullptr, nullptr, nullptr,
+                    nullptr));
+      BRTM_TRY(inject_source(c, it, jt, slot(j)));
     }
     for (int it = k1 - 1; it >= k0; --it) {
       if (it % jimg == 0) {
-        const float *snap = slot(it - k0 + 2);
+        const float *snap = slot(it - k0);
         bool fused = false;
         BRTM_TRY(step(c, FORM_VL, ACT_IMG, FMT_F32, lc, lo, lo, snap, nullptr, img, nullptr,
                       REC_INJECT, it, jt, nullptr, &fused));

@@ -116,6 +116,10 @@ hipError_t launch_axpby(float *y, const float *x, double a, double b, long n,
                         hipStream_t st);  // y = a*y + b*x
 hipError_t launch_mul(float *y, const float *x, long n, hipStream_t st);  // y *= x
 hipError_t launch_fill(float *y, float v, long n, hipStream_t st);
+// Zero the 4-cell halo (outside the update set) of nfields fields of
+// [n3][n2][n1] floats placed stride floats apart.
+hipError_t launch_zero_halo(float *p, long stride, int nfields, int n1, int n2, int n3,
+                            hipStream_t st);
 
 // Volume operators (compute grid [n3][n2][n1]).
 hipError_t launch_dz(int adj, const float *in, float *out, int n12, int n3, int add,

@@ -1289,6 +1289,34 @@ __global__ void k_mul(float *y, const float *__restrict__ x, long n) {
     y[i] *= x[i];
 }
 
+// Zero the cells outside the update set (the 4-cell halo every step leaves
+// untouched) of nfields fields of [n3][n2][n1] floats, stride apart: one
+// block row per plane (blockIdx.y), one field per blockIdx.z.
+__global__ void k_zero_halo(float *p, long stride, int n1, int n2, int n3) {
+  const int z = blockIdx.y;
+  float *f = p + (long)blockIdx.z * stride + (long)z * n1 * n2;
+  const bool slab = z < 4 || z >= n3 - 4;
+  const int rows = 8 * n1;
+  const int total = slab ? n1 * n2 : rows + (n2 - 8) * 8;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int y, x;
+    if (slab) {
+      y = t / n1;
+      x = t - y * n1;
+    } else if (t < rows) {
+      const int r = t / n1;
+      x = t - r * n1;
+      y = r < 4 ? r : n2 - 8 + r;
+    } else {
+      const int e = t - rows;
+      y = 4 + e / 8;
+      const int k = e - (e / 8) * 8;
+      x = k < 4 ? k : n1 - 8 + k;
+    }
+    f[(long)y * n1 + x] = 0.f;
+  }
+}
+
 __global__ void k_fill(float *y, float v, long n) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
@@ -1675,6 +1703,15 @@ hipError_t launch_axpby(float *y, const float *x, double a, double b, long n, hi
 
 hipError_t launch_mul(float *y, const float *x, long n, hipStream_t st) {
   k_mul<<<grid_for(n, 256), 256, 0, st>>>(y, x, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_halo(float *p, long stride, int nfields, int n1, int n2, int n3,
+                            hipStream_t st) {
+  if (nfields <= 0) return hipSuccess;
+  if (n1 < 9 || n2 < 9 || n3 < 9 || n3 > 65535 || nfields > 65535) return hipErrorInvalidValue;
+  const int gx = (8 * n1 + 8 * n2 + 255) / 256;
+  k_zero_halo<<<dim3(gx, n3, nfields), 256, 0, st>>>(p, stride, n1, n2, n3);
   return hipGetLastError();
 }
 

@@ -312,6 +312,28 @@ def test_rtm_adjoint_parity(dev, store):
     assert rel(img, io) < 1e-4
 
 
+@pytest.mark.parametrize("ckpt", [2, 3, 7, 13, 200])
+def test_checkpoint_adjoint_is_bit_identical_to_full_store(dev, ckpt):
+    """Checkpoints written by the propagation itself and a segment recompute
+    that starts from them give the same bits as storing every snapshot in
+    FP32 (the same arithmetic on the same inputs), and leave the last two
+    source fields as the plain propagation does."""
+    announce(f"checkpoint bit-exact ckpt={ckpt}")
+    P, r = make_shot(dev, nt=120, jimg=1)
+    d = r.standard_normal((len(P.rlocs), P.ntr_d))
+    d[P.rlocs < 0] = 0
+    out = {}
+    for store in ("f32", "ckpt"):
+        p0, p1 = dev.source_prop(1, P.nt, 1, store=store, ckpt=ckpt, get_last=True)
+        dev.set_receivers(P.rlocs, d)
+        dev.zero_image()
+        dev.rtm_adjoint(1, P.nt, 1)
+        out[store] = (dev.get_image(), p0, p1)
+    for a, b in zip(out["ckpt"], out["f32"]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out["f32"][0]).max() > 0
+
+
 @pytest.mark.parametrize("store", ["f32", "bf16", "f16", "ckpt"])
 def test_dot_product_gpu(dev, store):
     """<F m, d> = <m, F^T d> within 1e-4 with the FP32 GPU operator, for every
