@@ -240,22 +240,28 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
   BRTM_REQUIRE(c->nsrc == 0 || (nt - 1) / jt + 8 <= c->ntr_s, BRTM_EINVAL,
                "source traces shorter than (nt-1)/jt + 8 samples");
   Store &S = cur_store(c);
-  S.data.release();
-  S.scale.release();
+  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
+  const int fmt = fmt_of_store(kind);
+  // Checkpoints written straight by the steps (ckpt >= 2) need zero halos:
+  // a checkpoint buffer that already has them is kept, anything else is
+  // allocated afresh.
+  const bool ck_direct = kind == BRTM_STORE_CKPT && ckpt >= 2;
+  const int ck_count = kind == BRTM_STORE_CKPT ? (nt + ckpt - 1) / ckpt : 0;
+  const bool keep = ck_direct && S.kind == BRTM_STORE_CKPT && S.ckpt_halo_zero &&
+                    S.data.bytes >= 2 * fbytes * (size_t)ck_count;
+  if (!keep) S.release();
   S.kind = kind;
   S.nt = nt;
   S.jt = jt;
   S.jimg = jimg;
   S.ckpt = ckpt;
   S.count = 0;
-  const size_t fbytes = sizeof(float) * (size_t)c->ncell;
-  const int fmt = fmt_of_store(kind);
   if (fmt != FMT_NONE) {
     S.count = (nt - 1) / jimg + 1;
     BRTM_TRY(S.data.alloc(fmt_bytes(fmt) * (size_t)c->ncell * S.count));
     if (fmt != FMT_F32) BRTM_TRY(S.scale.alloc(sizeof(float) * (size_t)c->n3 * S.count));
   } else if (kind == BRTM_STORE_CKPT) {
-    S.count = (nt + ckpt - 1) / ckpt;
+    S.count = ck_count;
     BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
   }
   float *const wx = c->fb.as<float>(), *const wy = c->fa.as<float>();
@@ -263,7 +269,6 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
   // Checkpoint pair K = (s^{K ckpt}, s^{K ckpt - 1}).  With ckpt >= 2 the
   // two steps before each checkpoint write straight into its pair, and the
   // steps that read a pair write out of place into a work buffer.
-  const bool ck_direct = kind == BRTM_STORE_CKPT && ckpt >= 2;
   auto ck_cur = [&](int K) {
     return reinterpret_cast<float *>(S.data.as<char>() + 2 * fbytes * (size_t)K);
   };
@@ -271,8 +276,12 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     // the steps write only the update set: the pairs' halos must be zero
     // like the work buffers', and pair 0 is (s^0, s^-1) = 0
     BRTM_HIP(hipMemsetAsync(ck_cur(0), 0, 2 * fbytes, c->stream));
-    BRTM_HIP(launch_zero_halo(ck_cur(1), c->ncell, 2 * (S.count - 1), c->n1, c->n2, c->n3,
-                              c->stream));
+    if (!keep) {
+      const size_t nf = S.data.bytes / fbytes;  // whole buffer, also beyond count pairs
+      BRTM_HIP(launch_zero_halo(ck_cur(0) + 2 * c->ncell, c->ncell, (int)nf - 2, c->n1, c->n2,
+                                c->n3, c->stream));
+      S.ckpt_halo_zero = true;
+    }
     cur = ck_cur(0);
     old = ck_cur(0) + c->ncell;
   } else {
@@ -519,8 +528,7 @@ int brtm_destroy(brtm_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &s : c->stores) {
-    s.data.release();
-    s.scale.release();
+    s.release();
   }
   for (DevBuf *b : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
                     &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap,
@@ -579,8 +587,7 @@ int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, flo
   c->seg.release();
   c->seg_slots = 0;
   for (auto &s : c->stores) {
-    s.data.release();
-    s.scale.release();
+    s.release();
     s.kind = BRTM_STORE_NONE;
   }
   BRTM_HIP(hipStreamSynchronize(c->stream));
@@ -766,8 +773,7 @@ int brtm_select_shot(brtm_ctx *c, int ishot) {
 int brtm_release_store(brtm_ctx *c, int ishot) {
   BRTM_REQUIRE(c && ishot >= 0, BRTM_EINVAL, "bad shot index");
   if (ishot < (int)c->stores.size()) {
-    c->stores[ishot].data.release();
-    c->stores[ishot].scale.release();
+    c->stores[ishot].release();
     c->stores[ishot].kind = BRTM_STORE_NONE;
   }
   return BRTM_OK;
