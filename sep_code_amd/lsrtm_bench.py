@@ -1,6 +1,7 @@
 """LSRTM time-to-residual: plain CG against learned-preconditioned flexible CG.
 
     python -m sep_code_amd.lsrtm_bench --n 256 --nt 2500 [--diag-eps 0.01]
+    python -m sep_code_amd.lsrtm_bench --n 256 --nt 2500 --op-timing 2   # operator only
 
 All solvers run on the same operator (BF16 source snapshots, resident in
 HBM before the clock starts), from m = 0, until the misfit to the noise-free
@@ -57,6 +58,9 @@ def main():
     ap.add_argument("--split-eps", type=float, default=0.01,
                     help="> 0: the learned run is z = W P(W g), W = (I + eps max I)^-1/2 "
                          "(the network trained with the same split)")
+    ap.add_argument("--op-timing", type=int, default=0,
+                    help="> 0: only time that many Born forward / adjoint applications of "
+                         "the operator (wall clock per shot-step) and exit")
     a = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
 
@@ -64,6 +68,26 @@ def main():
     w = configs.lsrtm(seed=a.seed, n_side=a.n, nt=a.nt)
     dev = BornDevice(0)
     op, tprep = lsrtm.setup_operator(dev, w)
+    if a.op_timing > 0:
+        m, d = op.model_vector().random(1), op.data_vector()
+        steps = len(w.shots) * a.nt
+        res = {"workload": w.name, "grid": list(w.n), "nt": a.nt, "shots": len(w.shots),
+               "store": w.store, "jimg": w.jimg, "prepare_s": tprep, "forward_us": [],
+               "adjoint_us": []}
+        for _ in range(a.op_timing):
+            dev.sync()
+            t0 = time.perf_counter()
+            op.forward(False, m, d)
+            dev.sync()
+            t1 = time.perf_counter()
+            op.adjoint(False, m, d)
+            dev.sync()
+            t2 = time.perf_counter()
+            res["forward_us"].append((t1 - t0) / steps * 1e6)
+            res["adjoint_us"].append((t2 - t1) / steps * 1e6)
+        print(json.dumps(res), flush=True)
+        dev.close()
+        return
     d, floor, e = lsrtm.observed_data(op, w, keep_noise=True)
     log(f"{w.name}: {len(w.shots)} shots, nt={a.nt}, prepared in {tprep:.1f}s, "
         f"noise |e|/|d| = {floor:.4f}, setup {time.perf_counter() - t0:.1f}s")
