@@ -142,21 +142,26 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
   // receivers inside the step: four-cell ring kernel, one receiver plane
-  const bool in_step = rec != REC_NONE && fused && c->rec_fusion && c->rec_plane &&
-                       c->nrec > 0 && xl == 4 && ring && form == FORM_VL;
+  const bool src = rec == REC_SOURCE;
+  const bool in_step = rec != REC_NONE && fused && c->rec_fusion && xl == 4 && ring &&
+                       form == FORM_VL &&
+                       (src ? c->src_plane && c->nsrc > 0 : c->rec_plane && c->nrec > 0);
   if (in_step) {
-    P.rz = c->rec_z;
-    P.nrec = c->nrec;
+    P.rz = src ? c->src_z : c->rec_z;
+    P.nrec = src ? c->nsrc : c->nrec;
+    P.rsp = src ? c->ntr_s : 1;
+    P.rst = src ? 1 : c->nrec;
     P.rid = it / jt;
-    P.rmap = c->rmap.as<int>();
-    P.rtr = c->traces.as<float>();
+    P.rmap = src ? c->smap.as<int>() : c->rmap.as<int>();
+    P.rtr = src ? c->strc.as<float>() : c->traces.as<float>();
     P.rscale = rec == REC_EXTRACT ? rscale : nullptr;
-    std::memcpy(P.rsinc, &c->sinc[1][8 * (it % jt)], sizeof(P.rsinc));
+    std::memcpy(P.rsinc, &c->sinc[src ? 0 : 1][8 * (it % jt)], sizeof(P.rsinc));
   }
+  const int kmode = !in_step ? REC_NONE : (src ? REC_INJECT : rec);
   if (xl == 1 && form == FORM_VL && ring)
     BRTM_HIP(launch_step_ring(act, fmt, P, ty, c->stream));
   else if (xl == 4)
-    BRTM_HIP(launch_step_x4(act, fmt, P, ty, ring, c->stream, in_step ? rec : REC_NONE));
+    BRTM_HIP(launch_step_x4(act, fmt, P, ty, ring, c->stream, kmode));
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
@@ -304,9 +309,10 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
       else if (old != wx && old != wy)
         out = cur == wx ? wy : wx;                     // s^{it-1} is a checkpoint: keep it
     }
+    bool fused = false;
     BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, out, nullptr, nullptr, nullptr,
-                  nullptr));
-    BRTM_TRY(inject_source(c, it, jt, out));
+                  nullptr, REC_SOURCE, it, jt, nullptr, &fused));
+    if (!fused) BRTM_TRY(inject_source(c, it, jt, out));
     if (fmt != FMT_NONE && it % jimg == 0) {
       const int k = it / jimg;
       if (fmt == FMT_F32) {
@@ -374,9 +380,10 @@ int born_forward(brtm_ctx *c, int jt, int nt, int jimg, bool accumulate) {
   for (int it = 0; it < nt; ++it) {
     bool fused = false;
     if (live) {
+      bool sfused = false;
       BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, sc, so, so, nullptr, nullptr, nullptr,
-                    nullptr));
-      BRTM_TRY(inject_source(c, it, jt, so));
+                    nullptr, REC_SOURCE, it, jt, nullptr, &sfused));
+      if (!sfused) BRTM_TRY(inject_source(c, it, jt, so));
     }
     if (it % jimg == 0) {
       const int k = it / jimg;
@@ -460,9 +467,10 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
       const int j = it - k0;
       const float *p = j == 0 ? ck_p : slot(j - 1);
       const float *po = j == 0 ? ck_po : (j == 1 ? ck_p : slot(j - 2));
+      bool fused = false;
       BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, p, po, slot(j), nullptr, nullptr, nullptr,
-                    nullptr));
-      BRTM_TRY(inject_source(c, it, jt, slot(j)));
+                    nullptr, REC_SOURCE, it, jt, nullptr, &fused));
+      if (!fused) BRTM_TRY(inject_source(c, it, jt, slot(j)));
     }
     for (int it = k1 - 1; it >= k0; --it) {
       if (it % jimg == 0) {
@@ -531,7 +539,7 @@ int brtm_destroy(brtm_ctx *c) {
     s.release();
   }
   for (DevBuf *b : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
-                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap, &c->smap,
                     &c->traces})
     b->release();
   for (hipEvent_t e : c->ktime_ev) (void)hipEventDestroy(e);
@@ -638,6 +646,31 @@ int brtm_set_sinc_table(brtm_ctx *c, int which, int nsinc, int ntab, const float
   return BRTM_OK;
 }
 
+// When the active points (loc >= 0) all lie in one depth plane with at most
+// one per cell, *ok is set, *z names the plane and map [n2][n1] holds each
+// cell's point index (-1: none); otherwise *ok is cleared.
+static int plane_map(brtm_ctx *c, const int *locs, int n, bool *ok, int *z, DevBuf *map) {
+  const long n12 = (long)c->n1 * c->n2;
+  std::vector<int> m(n12, -1);
+  int pz = -1;
+  bool good = true;
+  for (int j = 0; j < n && good; ++j) {
+    const long l = locs[j];
+    if (l < 0) continue;
+    const int lz = (int)(l / n12);
+    if (pz < 0) pz = lz;
+    if (lz != pz || m[l % n12] >= 0) good = false;
+    else m[l % n12] = j;
+  }
+  *ok = good && pz >= 0;
+  *z = *ok ? pz : -1;
+  if (*ok) {
+    BRTM_TRY(map->alloc(sizeof(int) * (size_t)n12));
+    BRTM_HIP(h2d(c, map->ptr, m.data(), sizeof(int) * (size_t)n12));
+  }
+  return BRTM_OK;
+}
+
 static bool in_update_set(const brtm_ctx *c, long loc) {
   if (loc < 0 || loc >= c->ncell) return false;
   const int i1 = (int)(loc % c->n1), i2 = (int)((loc / c->n1) % c->n2),
@@ -663,6 +696,7 @@ int brtm_set_source(brtm_ctx *c, int npts, int ntr, const int *locs, const float
   BRTM_TRY(c->strc.alloc(sizeof(float) * (size_t)npts * ntr));
   BRTM_HIP(h2d(c, c->slocs.ptr, locs, sizeof(int) * npts));
   BRTM_HIP(h2d(c, c->strc.ptr, vals, sizeof(float) * (size_t)npts * ntr));
+  BRTM_TRY(plane_map(c, locs, npts, &c->src_plane, &c->src_z, &c->smap));
   return BRTM_OK;
 }
 
@@ -688,26 +722,7 @@ int brtm_set_receivers(brtm_ctx *c, int nx, int ny, int ntr, const int *locs, co
   BRTM_TRY(c->rlocs.alloc(sizeof(int) * nrec));
   BRTM_HIP(h2d(c, c->rlocs.ptr, c->h_rlocs.data(), sizeof(int) * nrec));
   // One plane, one receiver per cell: the step kernel can take them.
-  {
-    const long n12 = (long)c->n1 * c->n2;
-    std::vector<int> map(n12, -1);
-    int z = -1;
-    bool ok = true;
-    for (int j = 0; j < nrec && ok; ++j) {
-      const long l = c->h_rlocs[j];
-      if (l < 0) continue;
-      const int lz = (int)(l / n12);
-      if (z < 0) z = lz;
-      if (lz != z || map[l % n12] >= 0) ok = false;
-      else map[l % n12] = j;
-    }
-    c->rec_plane = ok && z >= 0;
-    c->rec_z = c->rec_plane ? z : -1;
-    if (c->rec_plane) {
-      BRTM_TRY(c->rmap.alloc(sizeof(int) * (size_t)n12));
-      BRTM_HIP(h2d(c, c->rmap.ptr, map.data(), sizeof(int) * (size_t)n12));
-    }
-  }
+  BRTM_TRY(plane_map(c, c->h_rlocs.data(), nrec, &c->rec_plane, &c->rec_z, &c->rmap));
   const size_t tb = sizeof(float) * (size_t)nrec * ntr;
   BRTM_TRY(c->traces.alloc(tb));
   if (!vals) {
@@ -971,7 +986,7 @@ int brtm_device_bytes(brtm_ctx *c, double *bytes) {
   BRTM_REQUIRE(c && bytes, BRTM_EINVAL, "NULL argument");
   double b = 0;
   for (DevBuf *x : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
-                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap,
+                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap, &c->smap,
                     &c->traces, &c->mvw, &c->rscale})
     b += (double)x->bytes;
   for (auto &s : c->stores) b += (double)s.data.bytes + (double)s.scale.bytes;

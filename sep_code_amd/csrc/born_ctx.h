@@ -105,6 +105,10 @@ struct brtm_ctx {
   bool rec_plane = false;
   int rec_z = -1;
   brtm::DevBuf rmap;
+  // the same for the source points (the source propagation's injection)
+  bool src_plane = false;
+  int src_z = -1;
+  brtm::DevBuf smap;
 
   std::vector<brtm::Store> stores;
   int shot = 0;
@@ -130,9 +134,11 @@ int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out);
 // acc += sum over the current shot's stored snapshots of s^2 (compute grid).
 int illumination(brtm_ctx *c, float *acc);
 Store &cur_store(brtm_ctx *c);
-// rec (RecMode): receivers handled by the step at time index it (jt steps
-// per sample, rscale for extraction) when *fused comes back true; the caller
-// runs inject_data / extract_data otherwise.
+// rec (RecMode, or REC_SOURCE for the source points): receivers handled by
+// the step at time index it (jt steps per sample, rscale for extraction)
+// when *fused comes back true; the caller runs inject_data / extract_data /
+// inject_source otherwise.
+constexpr int REC_SOURCE = 3;
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img_or_null, const float *m_or_null,
          int rec = REC_NONE, int it = 0, int jt = 1, const float *rscale = nullptr,

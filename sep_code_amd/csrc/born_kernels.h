@@ -41,9 +41,11 @@ struct StepParams {
   const float *sscale;  // [n3] slice scales (16-bit formats), else null
   const float *m;       // LV: Born model
   float cimg;
-  // receivers (RecMode): rmap [n2][n1] receiver index or -1 on plane rz;
-  // traces rtr[t * nrec + r] (time-major), time index rid, taps rsinc
+  // receivers (RecMode): rmap [n2][n1] point index or -1 on plane rz; the
+  // trace sample (r, t) at rtr[r * rsp + t * rst], time index rid, taps rsinc
+  // (the data receivers: rsp 1, rst nrec; the source points: rsp ntr, rst 1)
   int rz, nrec, rid;
+  long rsp, rst;
   const int *rmap;
   float *rtr;
   const float *rscale;

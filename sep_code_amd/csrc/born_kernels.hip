@@ -1082,13 +1082,13 @@ k_step_x4r(const StepParams P) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                   if (P.rsinc[k] != 0.f)
-                    acc = fmaf(P.rsinc[k], P.rtr[(long)(P.rid + k) * P.nrec + ri[j]], acc);
+                    acc = fmaf(P.rsinc[k], P.rtr[ri[j] * P.rsp + (P.rid + k) * P.rst], acc);
                 pn[j] += acc;
               } else {
                 const float v = P.rscale ? pn[j] * P.rscale[ri[j]] : pn[j];
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                  if (P.rsinc[k] != 0.f) P.rtr[(long)(P.rid + k) * P.nrec + ri[j]] += v * P.rsinc[k];
+                  if (P.rsinc[k] != 0.f) P.rtr[ri[j] * P.rsp + (P.rid + k) * P.rst] += v * P.rsinc[k];
               }
             }
           }
