@@ -183,12 +183,13 @@ int brtm_set_x_per_lane(brtm_ctx *ctx, int cells);
  * not for the one-cell kernel.  Same results. */
 int brtm_set_halo_ring(brtm_ctx *ctx, int on);
 
-/* Receivers inside the step (no counterpart in the reference): when the
- * active receivers share one depth plane with at most one receiver per
- * cell, the four-cell ring step injects the data (RTM adjoint) or extracts
- * it (Born modelling) on that plane itself instead of a separate launch per
- * step.  on = 1 (the default) allows it, 0 keeps the separate kernels.  The
- * same results, to FP32 rounding of the imaging term. */
+/* Receivers and sources inside the step (no counterpart in the reference):
+ * when the active receivers share one depth plane with at most one receiver
+ * per cell, the four-cell ring step injects the data (RTM adjoint) or
+ * extracts it (Born modelling) on that plane itself instead of a separate
+ * launch per step; the source points likewise.  on = 1 (the default) allows
+ * it, 0 keeps the separate kernels.  The same results, to FP32 rounding of
+ * the imaging term. */
 int brtm_set_receiver_fusion(brtm_ctx *ctx, int on);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
