@@ -1,9 +1,10 @@
 // 3-D convolutions of the learned preconditioner on gfx950 MFMA.
 //
 // Forward (and input-gradient) convolutions are implicit GEMMs: M = output
-// channels, N = 16 consecutive x positions per wave, K = 27 taps x input
-// channels in chunks of 32, one v_mfma_f32_16x16x32_bf16 per (16-channel
-// output tile, K chunk).  Operand lanes: lane l holds A[l%16][8(l/16)+i] and
+// channels, N = 16 consecutive x positions (of YT rows) per wave, K = the 27
+// taps x input channels flattened and cut in chunks of 32, one
+// v_mfma_f32_16x16x32_bf16 per (16-channel output tile, row, K chunk).
+// Operand lanes: lane l holds A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16] (i < 8), and receives D[4(l/16)+j][l%16] (j < 4)
 // (checked on the device against a host product).  In channels-last layout
 // the 8 K values of a lane are 8 consecutive channels of one position: one
@@ -37,55 +38,77 @@ int grid1(long n, int bs = 256) {
 }
 
 // ------------------------------------------------------------- forward GEMM
-template <int CIN_P, int COUT, bool RELU, bool OUTF32>
+// K runs over (tap, channel) pairs flattened, 32 per MFMA: a lane's 8 K
+// values are 8 consecutive channels of one tap (CIN is a multiple of 8), so
+// a 16-channel layer packs two taps into one MFMA instead of padding each tap
+// to 32.  A wave computes 16 x positions of YT consecutive rows: the weight
+// operand of a K chunk is loaded once for the YT row tiles.
+template <int CIN, int COUT, int YT, bool RELU, bool OUTF32>
 __global__ void __launch_bounds__(256)
-k_conv_fwd(const uint16_t *__restrict__ x, int xcs, int cin, const uint16_t *__restrict__ W,
+k_conv_fwd(const uint16_t *__restrict__ x, int xcs, const uint16_t *__restrict__ W,
            const float *__restrict__ bias, void *y, int ycs, int N, int D, int H, int Wd) {
+  static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+  constexpr int CIN_P = (CIN + 31) / 32 * 32;
   constexpr int MT = COUT / 16;
+  constexpr int KTOT = 27 * CIN;
+  constexpr int NCH = (KTOT + 31) / 32;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wt = Wd >> 4;
+  const int wt = Wd >> 4, hg = H / YT;
   const long tile = (long)blockIdx.x * 4 + wave;
-  if (tile >= (long)N * D * H * wt) return;  // whole wave; no barriers below
+  if (tile >= (long)N * D * hg * wt) return;  // whole wave; no barriers below
   const int xt = (int)(tile % wt);
   long r = tile / wt;
-  const int yy = (int)(r % H);
-  r /= H;
+  const int y0 = (int)(r % hg) * YT;
+  r /= hg;
   const int zz = (int)(r % D);
   const int nn = (int)(r / D);
   const int x0 = xt * 16;
   const int col = lane & 15, kq = lane >> 4;
-  f32x4 acc[MT];
+  f32x4 acc[MT][YT];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 27; ++tap) {
-    const int z2 = zz + tap / 9 - 1, y2 = yy + (tap / 3) % 3 - 1, x2 = x0 + col + tap % 3 - 1;
-    const bool ok = z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd;
-    const long pos = (((long)nn * D + z2) * H + y2) * Wd + x2;
-    const uint16_t *wtap = W + (long)tap * COUT * CIN_P;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int c0 = 0; c0 < CIN_P; c0 += 32) {
-      const int c = c0 + 8 * kq;
-      const bf16x8 b = (ok && c < cin) ? ld8(x + pos * xcs + c) : zero8();
+    for (int t = 0; t < YT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int k = ch * 32 + 8 * kq;
+    const bool kin = k < KTOT;
+    const int tap = kin ? k / CIN : 0;
+    const int c = k - tap * CIN;
+    const int z2 = zz + tap / 9 - 1, x2 = x0 + col + tap % 3 - 1;
+    const int dy = (tap / 3) % 3 - 1;
+    const bool okzx = kin && z2 >= 0 && z2 < D && x2 >= 0 && x2 < Wd;
+    const long base = ((long)nn * D + z2) * H;
+    bf16x8 a[MT];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const bf16x8 a = ld8(wtap + (mt * 16 + col) * CIN_P + c);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
-      }
+    for (int mt = 0; mt < MT; ++mt)
+      a[mt] = kin ? ld8(W + ((long)tap * COUT + mt * 16 + col) * CIN_P + c) : zero8();
+#pragma unroll
+    for (int t = 0; t < YT; ++t) {
+      const int y2 = y0 + t + dy;
+      const bool ok = okzx && y2 >= 0 && y2 < H;
+      const bf16x8 bv = ok ? ld8(x + ((base + y2) * Wd + x2) * xcs + c) : zero8();
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv, acc[mt][t], 0, 0, 0);
     }
   }
-  const long opos = (((long)nn * D + zz) * H + yy) * Wd + x0 + col;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
+  for (int t = 0; t < YT; ++t) {
+    const long opos = (((long)nn * D + zz) * H + y0 + t) * Wd + x0 + col;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int co = mt * 16 + 4 * kq + j;
-      float v = acc[mt][j];
-      if constexpr (OUTF32) {
-        static_cast<float *>(y)[opos * ycs + co] = v;
-      } else {
-        v += bias[co];
-        if (RELU) v = fmaxf(v, 0.f);
-        static_cast<uint16_t *>(y)[opos * ycs + co] = f2bf(v);
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co = mt * 16 + 4 * kq + j;
+        float v = acc[mt][t][j];
+        if constexpr (OUTF32) {
+          static_cast<float *>(y)[opos * ycs + co] = v;
+        } else {
+          v += bias[co];
+          if (RELU) v = fmaxf(v, 0.f);
+          static_cast<uint16_t *>(y)[opos * ycs + co] = f2bf(v);
+        }
       }
     }
   }
@@ -232,11 +255,29 @@ __global__ void k_first_fwd(const float *__restrict__ x, const float *__restrict
       const bool ok = z2 >= 0 && z2 < D && y2 >= 0 && y2 < H && x2 >= 0 && x2 < Wd;
       nb[tap] = ok ? x[p + ((long)(tap / 9 - 1) * H + (tap / 3) % 3 - 1) * Wd + tap % 3 - 1] : 0.f;
     }
-    for (int co = 0; co < cout; ++co) {
-      float s = b[co];
+    if (cout % 8 == 0 && ycs % 8 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0) {
+      // 8 channels per 16-byte store
+      for (int c0 = 0; c0 < cout; c0 += 8) {
+        uint32_t w[4];
 #pragma unroll
-      for (int tap = 0; tap < 27; ++tap) s = fmaf(W[tap * cout + co], nb[tap], s);
-      y[p * ycs + co] = f2bf(fmaxf(s, 0.f));
+        for (int h = 0; h < 4; ++h) {
+          float s0 = b[c0 + 2 * h], s1 = b[c0 + 2 * h + 1];
+#pragma unroll
+          for (int tap = 0; tap < 27; ++tap) {
+            s0 = fmaf(W[tap * cout + c0 + 2 * h], nb[tap], s0);
+            s1 = fmaf(W[tap * cout + c0 + 2 * h + 1], nb[tap], s1);
+          }
+          w[h] = (uint32_t)f2bf(fmaxf(s0, 0.f)) | ((uint32_t)f2bf(fmaxf(s1, 0.f)) << 16);
+        }
+        *reinterpret_cast<uint4 *>(y + p * ycs + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    } else {
+      for (int co = 0; co < cout; ++co) {
+        float s = b[co];
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) s = fmaf(W[tap * cout + co], nb[tap], s);
+        y[p * ycs + co] = f2bf(fmaxf(s, 0.f));
+      }
     }
   }
 }
@@ -284,7 +325,20 @@ __global__ void k_head_fwd(const uint16_t *__restrict__ y, int ycs, int c,
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
        p += (long)gridDim.x * blockDim.x) {
     float s = b[0] + x0[p];
-    for (int k = 0; k < c; ++k) s = fmaf(W[k], bf2f(y[p * ycs + k]), s);
+    if (c % 8 == 0 && ycs % 8 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0) {
+      // 8 channels per 16-byte load
+      for (int k0 = 0; k0 < c; k0 += 8) {
+        const uint4 u = *reinterpret_cast<const uint4 *>(y + p * ycs + k0);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          s = fmaf(W[k0 + 2 * h], __uint_as_float(w[h] << 16), s);
+          s = fmaf(W[k0 + 2 * h + 1], __uint_as_float(w[h] & 0xffff0000u), s);
+        }
+      }
+    } else {
+      for (int k = 0; k < c; ++k) s = fmaf(W[k], bf2f(y[p * ycs + k]), s);
+    }
     out[p] = s;
   }
 }
@@ -391,6 +445,27 @@ __global__ void k_up_fwd(const uint16_t *__restrict__ x, int xcs, uint16_t *y, i
   }
 }
 
+// The same with 16-byte moves of 8 channels (c, both strides and both
+// slice offsets multiples of 8 channels).
+__global__ void k_up_fwd8(const uint16_t *__restrict__ x, int xcs, uint16_t *y, int ycs, int c8,
+                          int N, int D, int H, int Wd) {
+  const long n = (long)N * 8 * D * H * Wd * c8;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % c8) * 8;
+    long r = i / c8;
+    const int xx = (int)(r % (2 * Wd));
+    long t = r / (2 * Wd);
+    const int yv = (int)(t % (2 * H));
+    t /= 2 * H;
+    const int zv = (int)(t % (2 * D));
+    const int nn = (int)(t / (2 * D));
+    const long q = (((long)nn * D + zv / 2) * H + yv / 2) * Wd + xx / 2;
+    *reinterpret_cast<uint4 *>(y + r * ycs + k) =
+        *reinterpret_cast<const uint4 *>(x + q * xcs + k);
+  }
+}
+
 __global__ void k_up_bwd(const float *__restrict__ dy, int dycs, float *dx, int dxcs, int c,
                          int N, int D, int H, int Wd) {
   const long n = (long)N * D * H * Wd * c;  // over coarse positions
@@ -438,21 +513,30 @@ __global__ void k_adam(float *p, const float *__restrict__ g, float *m, float *v
   }
 }
 
-template <int CIN_P, int COUT>
-hipError_t fwd_t(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, bool relu,
+template <int CIN, int COUT, int YT>
+hipError_t fwd_y(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, bool relu,
                  bool f32, hipStream_t st) {
-  const long tiles = (long)v.n * v.d * v.h * (v.w / 16);
+  const long tiles = (long)v.n * v.d * (v.h / YT) * (v.w / 16);
   const int grid = (int)((tiles + 3) / 4);
   if (f32)
-    k_conv_fwd<CIN_P, COUT, false, true><<<grid, 256, 0, st>>>(
-        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+    k_conv_fwd<CIN, COUT, YT, false, true><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
   else if (relu)
-    k_conv_fwd<CIN_P, COUT, true, false><<<grid, 256, 0, st>>>(
-        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+    k_conv_fwd<CIN, COUT, YT, true, false><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
   else
-    k_conv_fwd<CIN_P, COUT, false, false><<<grid, 256, 0, st>>>(
-        (const uint16_t *)x.p, x.cs, x.c, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
+    k_conv_fwd<CIN, COUT, YT, false, false><<<grid, 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, W, b, y.p, y.cs, v.n, v.d, v.h, v.w);
   return hipGetLastError();
+}
+
+// Four rows per wave where that still gives 1024 workgroups, else one.
+template <int CIN, int COUT>
+hipError_t fwd_t(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, bool relu,
+                 bool f32, hipStream_t st) {
+  const long blocks4 = (long)v.n * v.d * (v.h / 4) * (v.w / 16) / 4;
+  if (v.h % 4 == 0 && blocks4 >= 1024) return fwd_y<CIN, COUT, 4>(x, W, b, y, v, relu, f32, st);
+  return fwd_y<CIN, COUT, 1>(x, W, b, y, v, relu, f32, st);
 }
 
 template <int CIN_P, int COUT>
@@ -468,19 +552,24 @@ hipError_t wgrad_t(const Act &x, const Act &dy, float *dW, Vol v, hipStream_t st
 
 }  // namespace
 
-// The (cin_pad, cout) pairs of the network's convolutions (16->16, 16->32,
-// 32->32, 32->64, 64->64, 96->32, 48->16), of their transposes (input
-// gradients) and of their weight gradients.
+// The (cin_pad, cout) pairs of the network's weight gradients (for the
+// forward convolutions and their transposes).
 #define BRTM_CONV_PAIRS(X) \
   X(32, 16) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(64, 16) X(64, 32) X(32, 96) X(32, 48)
+
+// The (cin, cout) pairs of the forward convolutions (16->16, 16->32, 32->32,
+// 32->64, 64->64, 96->32, 48->16) and of the transposed ones (input
+// gradients: 32->16, 64->32, 32->96, 16->48).
+#define BRTM_CONV_FWD_PAIRS(X)                                                          \
+  X(16, 16) X(16, 32) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(48, 16) X(32, 16) X(64, 32) \
+  X(32, 96) X(16, 48)
 
 hipError_t conv3d_fwd(const Act &x, const uint16_t *W, const float *bias, Act y, Vol v,
                       bool relu, bool out_f32, hipStream_t st) {
   if (v.w % 16 || x.c % 8 || x.cs % 8 || y.c % 16) return hipErrorInvalidValue;
-  const int cp = cpad(x.c);
-#define X(CP, CO) \
-  if (cp == CP && y.c == CO) return fwd_t<CP, CO>(x, W, bias, y, v, relu, out_f32, st);
-  BRTM_CONV_PAIRS(X)
+#define X(CI, CO) \
+  if (x.c == CI && y.c == CO) return fwd_t<CI, CO>(x, W, bias, y, v, relu, out_f32, st);
+  BRTM_CONV_FWD_PAIRS(X)
 #undef X
   return hipErrorInvalidValue;
 }
@@ -569,6 +658,14 @@ hipError_t avgpool_bwd(const float *dy, int dy_cs, float *dx, int dx_cs, int c, 
 }
 
 hipError_t upsample_fwd(const Act &x, Act y, Vol v, hipStream_t st) {
+  const bool vec = x.c % 8 == 0 && x.cs % 8 == 0 && y.cs % 8 == 0 &&
+                   reinterpret_cast<uintptr_t>(x.p) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(y.p) % 16 == 0;
+  if (vec) {
+    k_up_fwd8<<<grid1((long)v.n * 8 * v.spatial() * (x.c / 8)), 256, 0, st>>>(
+        (const uint16_t *)x.p, x.cs, (uint16_t *)y.p, y.cs, x.c / 8, v.n, v.d, v.h, v.w);
+    return hipGetLastError();
+  }
   k_up_fwd<<<grid1((long)v.n * 8 * v.spatial() * x.c), 256, 0, st>>>(
       (const uint16_t *)x.p, x.cs, (uint16_t *)y.p, y.cs, x.c, v.n, v.d, v.h, v.w);
   return hipGetLastError();

@@ -164,3 +164,62 @@ class UNet:
 
 def _vec_data(v: DeviceVector):
     return v.data_ptr
+
+
+# Multiply-adds of the network per position of a 64^3 patch, level by level
+# (3x3x3 convolutions; the 1-channel first layer and the 1x1x1 head aside):
+# level 1 (full resolution) e1b 16->16, d1a 48->16, d1b 16->16; level 2
+# (1/8 of the positions) e2a 16->32, e2b 32->32, d2a 96->32, d2b 32->32;
+# level 3 (1/64) b1 32->64, b2 64->64.
+CONV_MACS_PER_POSITION = 27 * ((16 * 16 + 48 * 16 + 16 * 16)
+                               + (16 * 32 + 32 * 32 + 96 * 32 + 32 * 32) / 8
+                               + (32 * 64 + 64 * 64) / 64)
+
+
+def main(argv=None):
+    """Time the whole-volume application (the preconditioner's cost per
+    flexible-CG iteration) on a random volume:
+        python -m sep_code_amd.unet --dims 206 --reps 3
+    prints one JSON line with ms per application and the convolutions'
+    useful TFLOP/s (patch positions, overlap included)."""
+    import argparse
+    import json
+    import os
+    import time
+
+    from .device import BornDevice
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dims", type=int, default=206)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--overlap", type=int, default=8)
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                       "weights", "precond_split_e0.01.npy"))
+    a = ap.parse_args(argv)
+    dims = (a.dims,) * 3
+    dev = BornDevice(0)
+    dev.create_space((10.0, 10.0, 10.0), 8.0, 0.0, (32, 32, 32))
+    net = UNet(dev, batch=a.batch, patch=64, seed=1)
+    net.load(a.weights)
+    g = DeviceVector(dev, int(np.prod(dims))).random(3)
+    z = DeviceVector(dev, g.n)
+    stride = 64 - 2 * a.overlap
+    per_axis = -(-a.dims // stride)  # patch origins every stride cells (brtm_unet_apply)
+    npatch = per_axis ** 3
+    net.apply(g, z, dims, overlap=a.overlap)
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        net.apply(g, z, dims, overlap=a.overlap)
+    dev.sync()
+    ms = (time.perf_counter() - t0) / a.reps * 1e3
+    flop = 2.0 * CONV_MACS_PER_POSITION * 64 ** 3 * npatch
+    print(json.dumps({"dims": list(dims), "patches": npatch, "batch": a.batch, "ms": round(ms, 2),
+                      "conv_tflop": round(flop / 1e12, 3),
+                      "conv_tflops": round(flop / (ms * 1e-3) / 1e12, 1)}), flush=True)
+    net.close()
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()
