@@ -95,3 +95,34 @@ def test_cli_programs_match_oracle(tmp_path):
     e = rel(inv2, m_r)
     announce(f"  regularised, muted LSRTM model rel err {e:.2e}")
     assert e < 1e-3 and 0 < start.min() and start.max() < at.n
+
+
+@pytest.mark.parametrize("precond", ["diag", "learned"])
+def test_cli_inv_preconditioned(tmp_path, precond):
+    """inv with the illumination-diagonal or the learned preconditioner (the
+    shipped network, flexible CG) runs on SEP files and lowers the misfit."""
+    vel, vaxes, daxes = write_small_survey(tmp_path)
+    m1, m2, m3 = (a.n for a in vaxes)
+    model = np.random.default_rng(5).standard_normal((m3, m2, m1)).astype(np.float32)
+    sep.write(str(tmp_path / "model.H"), model, vaxes)
+    t = str(tmp_path)
+    common = [f"velocity={t}/vel.H", f"wavelet={t}/wav.H", "bc_a=8", "src_depth=20",
+              "rec_depth=10", "store=f32"]
+    assert rtm_cli.main(["fwd", *common, f"data={t}/data.H", f"image={t}/model.H",
+                         f"dataout={t}/dout.H"]) == 0
+    announce(f"cli inv precond={precond}")
+    assert rtm_cli.main(["inv", *common, f"data={t}/dout.H", f"image={t}/inv.H", "niter=4",
+                         f"precond={precond}"]) == 0
+    inv, _ = sep.read(f"{t}/inv.H")
+    assert np.isfinite(inv).all() and np.abs(inv).max() > 0
+    sp = rtm_cli.problem_from_sep(sep.Params(common + [f"data={t}/data.H"]), "f32")
+    w = sp.workload
+    shots = [(s.src_locs, s.src_traces, s.rec_locs) for s in w.shots]
+    orc = O.CompositeBorn(w.n, w.d, w.V.astype(np.float64), w.bc_a, w.bc_b, shots, w.nt,
+                          w.dt * w.jt, jt=w.jt, jimg=w.jimg, cimg=w.cimg,
+                          model_box=sp.model_box)
+    dout, _ = sep.read(f"{t}/dout.H")
+    d = dout.astype(np.float64).ravel()
+    res = np.linalg.norm(orc.forward(inv).ravel() - d) / np.linalg.norm(d)
+    announce(f"  relative misfit after 4 iterations {res:.3e}")
+    assert res < 0.8
