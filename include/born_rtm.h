@@ -192,6 +192,12 @@ int brtm_set_halo_ring(brtm_ctx *ctx, int on);
  * the imaging term. */
 int brtm_set_receiver_fusion(brtm_ctx *ctx, int on);
 
+/* Global-load lookahead of the plain four-cell ring step, in z slices: 1
+ * (inputs of slice z+1 loaded while slice z computes), 2 (two slices ahead,
+ * 36 more registers per lane) or 0 (the default: the faster of the two on
+ * MI355X).  No counterpart in the reference; the same results. */
+int brtm_set_prefetch(brtm_ctx *ctx, int slices);
+
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms
  * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE

@@ -80,6 +80,9 @@ static hipError_t h2d(brtm_ctx *c, void *dst, const void *src, size_t bytes) {
   return e;
 }
 
+// Load lookahead of the plain four-cell ring step when left automatic.
+constexpr int kAutoPrefetch = 1;
+
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img, const float *m, int rec, int it,
          int jt, const float *rscale, bool *fused) {
@@ -100,6 +103,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   P.sscale = sscale;
   P.m = m;
   P.cimg = c->cimg;
+  P.pf = c->prefetch == 0 ? kAutoPrefetch : c->prefetch;
   // Tile rows / workgroup target per variant (measured on MI355X at 512^3):
   // plain steps prefer tall tiles, 16-bit snapshot variants short ones.
   int ty = c->ty_tile, tb = c->target_blocks;
@@ -902,6 +906,13 @@ int brtm_set_tuning(brtm_ctx *c, int ty_tile, int target_blocks) {
   BRTM_REQUIRE(target_blocks >= 1, BRTM_EINVAL, "target_blocks must be >= 1");
   c->ty_tile = ty_tile;
   c->target_blocks = target_blocks;
+  return BRTM_OK;
+}
+
+int brtm_set_prefetch(brtm_ctx *c, int slices) {
+  BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
+  BRTM_REQUIRE(slices >= 0 && slices <= 2, BRTM_EINVAL, "prefetch: 0 (automatic), 1 or 2 slices");
+  c->prefetch = slices;
   return BRTM_OK;
 }
 

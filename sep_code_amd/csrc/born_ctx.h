@@ -82,6 +82,7 @@ struct brtm_ctx {
   int x_per_lane = 0;  // VL steps: 0 automatic, 1, 2 (n1 even) or 4 (n1 % 4 == 0)
   int halo_ring = -1;  // six-plane LDS ring: -1 automatic (on for 4 cells per lane), 0, 1
   bool rec_fusion = true;  // receivers inside the four-cell ring step where they allow it
+  int prefetch = 0;        // load lookahead of the plain four-cell ring step: 0 auto, 1, 2
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)

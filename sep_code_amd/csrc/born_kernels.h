@@ -46,6 +46,9 @@ struct StepParams {
   // (the data receivers: rsp 1, rst nrec; the source points: rsp ntr, rst 1)
   int rz, nrec, rid;
   long rsp, rst;
+  // slices of global-load lookahead of the four-cell ring step (1, or 2 for
+  // the plain step)
+  int pf;
   const int *rmap;
   float *rtr;
   const float *rscale;

@@ -890,10 +890,12 @@ k_step_x4(const StepParams P) {
 // 149 KB of LDS, which the one workgroup per CU that the registers allow
 // can have.
 // ---------------------------------------------------------------------------
-template <int TY, int RPT, int FMT, int ACT, int REC = REC_NONE>
+template <int TY, int RPT, int FMT, int ACT, int REC = REC_NONE, int PF = 1>
 __global__ void __launch_bounds__(64 * (TY / RPT))
 k_step_x4r(const StepParams P) {
   static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  static_assert(PF == 1 || (PF == 2 && ACT == ACT_NONE),
+                "two slices of prefetch only for the plain step (registers)");
   static_assert(!(REC == REC_INJECT && ACT == ACT_BORN) && !(REC == REC_EXTRACT && ACT == ACT_IMG),
                 "injection goes with the adjoint, extraction with Born modelling");
   constexpr int NWV = TY / RPT;
@@ -1005,6 +1007,20 @@ k_step_x4r(const StepParams P) {
   load_halo(z0 + 4, hy, hx);
   f32x4 cpo[RPT], cV[RPT], cS[RPT], cX[RPT];
   load_center(z0, cpo, cV, cS, cX);
+  // PF = 2: the inputs of slice z+1 were loaded one iteration earlier and
+  // wait in the p* registers while the loads of slice z+2 are in flight
+  f32x4 pq[RPT], phy[HY], phx[RPT], ppo[RPT], pV[RPT], pS[RPT], pX[RPT];
+  if constexpr (PF == 2) {
+    if (z0 + 1 < z1) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int y = yb + r;
+        pq[r] = ld4(P.p, (z0 + 5) * s3 + y * n1 + x, xmem && y < n2);
+      }
+      load_halo(z0 + 5, phy, phx);
+      load_center(z0 + 1, ppo, pV, pS, pX);
+    }
+  }
 
   const float c0 = P.c[0];
   for (int z = z0; z < z1; ++z) {
@@ -1015,16 +1031,17 @@ k_step_x4r(const StepParams P) {
       put_plane(z + 4, c, hy, hx);
     }
     const int zn = z + 1;
+    const int zf = z + PF;  // the slice whose inputs are loaded now
     f32x4 nq[RPT], nhy[HY], nhx[RPT];
     f32x4 npo[RPT], nV[RPT], nS[RPT], nX[RPT];
-    if (zn < z1) {
+    if (zf < z1) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int y = yb + r;
-        nq[r] = ld4(P.p, (zn + 4) * s3 + y * n1 + x, xmem && y < n2);
+        nq[r] = ld4(P.p, (zf + 4) * s3 + y * n1 + x, xmem && y < n2);
       }
-      load_halo(zn + 4, nhy, nhx);
-      load_center(zn, npo, nV, nS, nX);
+      load_halo(zf + 4, nhy, nhx);
+      load_center(zf, npo, nV, nS, nX);
     }
     __syncthreads();
 
@@ -1108,15 +1125,33 @@ k_step_x4r(const StepParams P) {
       for (int r = 0; r < RPT; ++r) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[r][k] = q[r][k + 1];
-        q[r][8] = nq[r];
-        cpo[r] = npo[r];
-        cV[r] = nV[r];
-        cS[r] = nS[r];
-        cX[r] = nX[r];
-        hx[r] = nhx[r];
+        if constexpr (PF == 2) {
+          q[r][8] = pq[r];
+          cpo[r] = ppo[r];
+          cV[r] = pV[r];
+          hx[r] = phx[r];
+          pq[r] = nq[r];
+          ppo[r] = npo[r];
+          pV[r] = nV[r];
+          phx[r] = nhx[r];
+        } else {
+          q[r][8] = nq[r];
+          cpo[r] = npo[r];
+          cV[r] = nV[r];
+          cS[r] = nS[r];
+          cX[r] = nX[r];
+          hx[r] = nhx[r];
+        }
       }
 #pragma unroll
-      for (int j = 0; j < HY; ++j) hy[j] = nhy[j];
+      for (int j = 0; j < HY; ++j) {
+        if constexpr (PF == 2) {
+          hy[j] = phy[j];
+          phy[j] = nhy[j];
+        } else {
+          hy[j] = nhy[j];
+        }
+      }
     }
   }
 }
@@ -1503,6 +1538,16 @@ hipError_t launch_x4_kernel(const StepParams &P, hipStream_t st, int rec) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.nty * P.ntz);
   if constexpr (RING) {
+    if constexpr (ACT == ACT_NONE) {
+      if (P.pf == 2 && rec == REC_NONE) {
+        k_step_x4r<TY, RPT, FMT, ACT, REC_NONE, 2><<<grid, block, 0, st>>>(P);
+        return hipGetLastError();
+      }
+      if (P.pf == 2 && rec == REC_INJECT) {
+        k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT, 2><<<grid, block, 0, st>>>(P);
+        return hipGetLastError();
+      }
+    }
     if (rec == REC_NONE) {
       k_step_x4r<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
     } else if (rec == REC_INJECT) {

@@ -189,6 +189,11 @@ class BornDevice:
         1 on, 0 off, -1 automatic (the default: on for four cells per lane)."""
         call("brtm_set_halo_ring", self.h, -1 if on is None or on == -1 else int(bool(on)))
 
+    def set_prefetch(self, slices):
+        """Load lookahead of the plain four-cell ring step: 1 or 2 z slices,
+        0 automatic (the default)."""
+        call("brtm_set_prefetch", self.h, int(slices))
+
     def set_receiver_fusion(self, on):
         """Receivers injected / extracted inside the four-cell ring step when
         they share one plane (1, the default) or by separate kernels (0)."""

@@ -10,8 +10,10 @@ kernels' counters), then --reps launches each of the stencil variants
 and prints their HIP-event mean durations and algorithmic bandwidths, for each
 cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 8-byte accesses; 4: 256-wide tiles with 16-byte accesses; names get an
-"_x2" / "_x4" suffix) and halo-ring setting in --ring (1 on, suffix "_ring";
-0 off, suffix "_noring"; -1 the library's automatic choice, no suffix).
+"_x2" / "_x4" suffix), halo-ring setting in --ring (1 on, suffix "_ring";
+0 off, suffix "_noring"; -1 the library's automatic choice, no suffix) and
+load lookahead in --prefetch (1 or 2 slices, suffix "_pf1" / "_pf2"; 0
+automatic, no suffix).
 """
 from __future__ import annotations
 
@@ -36,6 +38,8 @@ def main():
     ap.add_argument("--variants", default="all")
     ap.add_argument("--xl", default="1,2")
     ap.add_argument("--ring", default="-1", help="comma list of halo-ring settings (-1, 0, 1)")
+    ap.add_argument("--prefetch", default="0",
+                    help="comma list of load lookaheads of the plain ring step (0 auto, 1, 2)")
     a = ap.parse_args()
     n = a.n
     dev = BornDevice(0)
@@ -50,14 +54,16 @@ def main():
         dev.set_halo_ring(ring)
         for xl in [int(v) for v in a.xl.split(",")]:
             dev.set_x_per_lane(xl)
-            for name, form, act, fmt, B in VARIANTS:
-                if a.variants != "all" and name not in a.variants.split(","):
-                    continue
-                ms = dev.time_step(form, act, fmt, a.reps)
-                key = name + (f"_x{xl}" if xl != 1 else "") + {1: "_ring", 0: "_noring",
-                                                                  -1: ""}[ring]
-                out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
-                            "gpts": n ** 3 / ms / 1e6}
+            for pf in [int(v) for v in a.prefetch.split(",")]:
+                dev.set_prefetch(pf)
+                for name, form, act, fmt, B in VARIANTS:
+                    if a.variants != "all" and name not in a.variants.split(","):
+                        continue
+                    ms = dev.time_step(form, act, fmt, a.reps)
+                    key = (name + (f"_x{xl}" if xl != 1 else "") +
+                           {1: "_ring", 0: "_noring", -1: ""}[ring] + (f"_pf{pf}" if pf else ""))
+                    out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
+                                "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()
 

@@ -67,13 +67,15 @@ def test_step_parity(dev, n, form, ty):
     dev.set_tuning(0, 2048)
 
 
-VARIANTS = [(2, 0), (4, 0), (1, 1), (4, 1)]  # (cells per lane, halo ring)
-VARIANT_IDS = ["x2", "x4", "ring", "x4ring"]
+# (cells per lane, halo ring[, load lookahead of the plain ring step])
+VARIANTS = [(2, 0), (4, 0), (1, 1), (4, 1, 1), (4, 1, 2)]
+VARIANT_IDS = ["x2", "x4", "ring", "x4ring", "x4ring_pf2"]
 
 
 def set_variant(dev, v):
     dev.set_x_per_lane(v[0])
     dev.set_halo_ring(v[1])
+    dev.set_prefetch(v[2] if len(v) > 2 else 0)
 
 
 @pytest.mark.parametrize("v", VARIANTS, ids=VARIANT_IDS)
