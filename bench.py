@@ -20,7 +20,9 @@ the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 
 cpu_baseline: the FP32 OpenMP CPU restatement (oracle/liborc32.so) of the
 same stencil step at 512^3, timed on the host cores: the median of three
-samples of ~3.3 s each.
+samples of ~3.3 s each.  With --workload rtm128 (128^3, nt = 1000) the
+whole shot runs on the CPU instead, three times, and the line also carries
+the relative difference of the CPU image from the GPU image.
 """
 from __future__ import annotations
 
@@ -89,13 +91,19 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(w, seconds=10.0, runs=3):
-    """FP32 OpenMP restatement of the VL step at the workload's size: the
-    median per-step rate of `runs` timed samples of seconds/runs each."""
+def _omp_threads():
+    """Host threads for the CPU restatement; set before libgomp loads."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    return threads
+
+
+def cpu_baseline(w, seconds=10.0, runs=3):
+    """FP32 OpenMP restatement of the VL step at the workload's size: the
+    median per-step rate of `runs` timed samples of seconds/runs each."""
+    threads = _omp_threads()
     from oracle import born_oracle as O  # loads libgomp after the settings above
     P = O.Problem(w.n, w.d, w.V, w.bc_a, w.bc_b, prec=32)
     rng = np.random.default_rng(0)
@@ -104,27 +112,73 @@ def cpu_baseline(w, seconds=10.0, runs=3):
     v = cur.reshape(w.n[2], w.n[1], w.n[0])
     v[4:-4, 4:-4, 4:-4] = rng.standard_normal((w.n[2] - 8, w.n[1] - 8, w.n[0] - 8))
     import ctypes as C
-    P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)  # warm-up
-    rates, total_steps, total_s = [], 0, 0.0
-    for _ in range(runs):
-        steps, t0 = 0, time.perf_counter()
-        while True:
-            P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)
-            steps += 1
-            el = time.perf_counter() - t0
-            if (el >= seconds / runs and steps >= 2) or steps >= 1000:
-                break
-        rates.append(w.ncell * steps / el / 1e9)
-        total_steps += steps
-        total_s += el
+    P.L.orc_flush_denormals(1)
+    try:
+        P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)  # warm-up
+        rates, total_steps, total_s = [], 0, 0.0
+        for _ in range(runs):
+            steps, t0 = 0, time.perf_counter()
+            while True:
+                P.L.orc_propagate(C.byref(P.s), 0, 1, cur.ctypes.data, old.ctypes.data)
+                steps += 1
+                el = time.perf_counter() - t0
+                if (el >= seconds / runs and steps >= 2) or steps >= 1000:
+                    break
+            rates.append(w.ncell * steps / el / 1e9)
+            total_steps += steps
+            total_s += el
+    finally:
+        P.L.orc_flush_denormals(0)
     gpts = float(np.median(rates))
     return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": threads, "kind": "port",
             "cpu": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"median of {runs} samples ({total_steps} FP32 VL stencil steps with "
                       f"fused sponge in all, {total_s:.1f} s) of the {w.n[0]}x{w.n[1]}x"
                       f"{w.n[2]} grid on {threads} OpenMP threads (OMP_PROC_BIND="
-                      f"{os.environ['OMP_PROC_BIND']}, OMP_PLACES={os.environ['OMP_PLACES']}); "
-                      "per-step rate, not the full 4000-step shot"}
+                      f"{os.environ['OMP_PROC_BIND']}, OMP_PLACES={os.environ['OMP_PLACES']}, denormals "
+                      "flushed); per-step rate, not the full 4000-step shot"}
+
+
+def cpu_full_shot(w, D, img_gpu, runs=3):
+    """The whole RTM shot on the FP32 OpenMP restatement, the workload's own
+    algorithm: FP32 snapshots of every imaging step, or checkpoints with
+    segment recompute.  Same source, receivers and Born data as the GPU run;
+    reports the median RTM Gpts/s of `runs` shots and the relative
+    difference of the CPU image from the GPU image."""
+    threads = _omp_threads()
+    from oracle import born_oracle as O  # loads libgomp after the settings above
+    shot = w.shots[0]
+    P = O.Problem(w.n, w.d, w.V, w.bc_a, w.bc_b, src_locs=shot.src_locs,
+                  src_tr=shot.src_traces, rec_locs=shot.rec_locs, ntr_d=w.ntr, jt=w.jt,
+                  nt=w.nt, jimg=w.jimg, cimg=w.cimg, prec=32)
+    D = np.ascontiguousarray(D, np.float32).reshape(len(shot.rec_locs), w.ntr)
+    times, img = [], None
+    P.L.orc_flush_denormals(1)
+    try:
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            if w.store == "f32":
+                snaps, _ = P.source_prop(True)
+                img = P.rtm_adjoint(D, snaps)
+                del snaps
+            else:
+                img = P.rtm_adjoint(D, None, ckpt=w.ckpt)
+            times.append(time.perf_counter() - t0)
+    finally:
+        P.L.orc_flush_denormals(0)
+    el = float(np.median(times))
+    g = np.asarray(img_gpu, np.float64).ravel()
+    diff = float(np.linalg.norm(np.asarray(img, np.float64) - g) / max(np.linalg.norm(g), 1e-300))
+    algo = ("FP32 snapshots of every imaging step" if w.store == "f32"
+            else f"checkpoints every {w.ckpt} steps and segment recompute")
+    return {"value": round(2.0 * w.ncell * w.nt / el / 1e9, 4), "unit": "Gpts/s",
+            "cores": threads, "kind": "port", "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"the full {w.nt}-step RTM shot of the {w.n[0]}x{w.n[1]}x{w.n[2]} "
+                      f"grid ({algo}), median of {runs} shots ({el:.2f} s each) on {threads} "
+                      f"OpenMP threads (OMP_PROC_BIND={os.environ['OMP_PROC_BIND']}, "
+                      f"OMP_PLACES={os.environ['OMP_PLACES']}, denormals flushed); same metric as "
+                      "value",
+            "image_rel_diff_vs_gpu": float(f"{diff:.3e}")}
 
 
 def main():
@@ -141,6 +195,9 @@ def main():
                          "4 at 512^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the whole shot on the CPU restatement (the default for "
+                         "rtm128; at 512^3 it takes hours)")
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
                     help="HBM bytes per stencil launch measured with rocprofv3 --pmc")
     a = ap.parse_args()
@@ -201,15 +258,20 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(w, a.cpu_seconds)
+        if a.cpu_full or w.name == "rtm128":
+            cpu = cpu_full_shot(w, dev.get_receivers(), img)
+        else:
+            cpu = cpu_baseline(w, a.cpu_seconds)
     out = {
         "metric": "RTM Gpts/s (% HBM roofline); LSRTM time-to-1e-2 residual, learned vs plain CG",
         "value": round(value, 3), "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{w.name}: RTM shot, {w.n[0]}x{w.n[1]}x{w.n[2]} FP32, "
-                               f"nt={w.nt}, source checkpoints every {w.ckpt} steps, "
-                               "imaging every step",
+                               f"nt={w.nt}, " + (f"source checkpoints every {w.ckpt} steps"
+                                                 if w.store == "ckpt" else
+                                                 f"{w.store} source snapshots") +
+                               ", imaging every step",
                    "grid": list(w.n), "nt": w.nt, "store": w.store, "ckpt": w.ckpt,
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "image_finite": finite,

@@ -76,7 +76,7 @@ def lib(prec: int = 64) -> C.CDLL:
     sfx = "_f64" if prec == 64 else "_f32"
     for name in ("orc_step_vl", "orc_step_lv", "orc_source_prop", "orc_born_forward",
                  "orc_rtm_adjoint", "orc_propagate", "orc_coeffs", "orc_sponge_table",
-                 "orc_inject", "orc_extract", "orc_dot"):
+                 "orc_inject", "orc_extract", "orc_dot", "orc_flush_denormals"):
         setattr(L, name, getattr(L, name + sfx))
     P, I, R = C.c_void_p, C.c_int, (C.c_double if prec == 64 else C.c_float)
     argtypes = {
@@ -96,6 +96,8 @@ def lib(prec: int = 64) -> C.CDLL:
         getattr(L, name).argtypes = at
         getattr(L, name).restype = C.c_int
     L.orc_dot.restype = C.c_double
+    L.orc_flush_denormals.argtypes = [I]
+    L.orc_flush_denormals.restype = None
     L.orc_coeffs.restype = None
     _LIBS[prec] = L
     return L
