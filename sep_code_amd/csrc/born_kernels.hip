@@ -1228,21 +1228,34 @@ __global__ void k_receiver_scale(const float *__restrict__ V, const float *__res
 // ---------------------------------------------------------------------------
 // Snapshot codec: scale = max |s| over each z slice; q = s / scale.
 // ---------------------------------------------------------------------------
-__global__ void k_slice_absmax(const float *__restrict__ s, unsigned int *smax, int n12) {
+// 256 threads (4 waves) per block; blockIdx.y is the slice.  A slice whose
+// length is a multiple of 4 is read 16 bytes per lane (hipMalloc'ed fields
+// are 16-byte aligned, so every slice is).  The maximum is exact, so the
+// result does not depend on the order of the reduction.
+__global__ void __launch_bounds__(256) k_slice_absmax(const float *__restrict__ s,
+                                                      unsigned int *smax, int n12) {
   const int z = blockIdx.y;
   const float *sl = s + (long)z * n12;
+  const int stride = gridDim.x * blockDim.x;
   float m = 0.f;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += gridDim.x * blockDim.x)
-    m = fmaxf(m, fabsf(sl[i]));
-  __shared__ float red[256];
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
+  if ((n12 & 3) == 0) {
+    const f32x4 *v = reinterpret_cast<const f32x4 *>(sl);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n12 >> 2); i += stride) {
+      const f32x4 a = v[i];
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
+    }
+  } else {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n12; i += stride)
+      m = fmaxf(m, fabsf(sl[i]));
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
   // Non-negative floats order like their bit patterns.
-  if (threadIdx.x == 0) atomicMax(smax + z, __float_as_uint(red[0]));
+  if (threadIdx.x == 0)
+    atomicMax(smax + z, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
 }
 
 template <int FMT>
@@ -1706,9 +1719,12 @@ hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int 
                          hipStream_t st) {
   hipError_t e = hipMemsetAsync(sscale, 0, sizeof(float) * n3, st);
   if (e != hipSuccess) return e;
+  // absmax: 16 values per lane per pass
+  int ga = (n12 + 4095) / 4096;
+  if (ga > 64) ga = 64;
+  k_slice_absmax<<<dim3(ga, n3), 256, 0, st>>>(s, reinterpret_cast<unsigned int *>(sscale), n12);
   int gx = (n12 + 255) / 256;
   if (gx > 64) gx = 64;
-  k_slice_absmax<<<dim3(gx, n3), 256, 0, st>>>(s, reinterpret_cast<unsigned int *>(sscale), n12);
   if (fmt == FMT_BF16)
     k_encode<FMT_BF16><<<dim3(gx, n3), 256, 0, st>>>(s, dst, sscale, n12);
   else if (fmt == FMT_F16)
