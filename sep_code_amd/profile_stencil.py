@@ -13,7 +13,8 @@ cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 "_x2" / "_x4" suffix), halo-ring setting in --ring (1 on, suffix "_ring";
 0 off, suffix "_noring"; -1 the library's automatic choice, no suffix) and
 load lookahead in --prefetch (1 or 2 slices, suffix "_pf1" / "_pf2"; 0
-automatic, no suffix).
+automatic, no suffix).  With --ty, every workgroup target in --tb is timed
+(suffix "_ty<rows>_tb<target>").
 """
 from __future__ import annotations
 
@@ -34,7 +35,10 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--copy-gib", type=float, default=1.0)
-    ap.add_argument("--ty", type=int, default=0)
+    ap.add_argument("--ty", type=int, default=0,
+                    help="tile rows (0: the library's per-variant choice)")
+    ap.add_argument("--tb", default="2048",
+                    help="comma list of workgroup targets (used only with --ty)")
     ap.add_argument("--variants", default="all")
     ap.add_argument("--xl", default="1,2")
     ap.add_argument("--ring", default="-1", help="comma list of halo-ring settings (-1, 0, 1)")
@@ -43,7 +47,8 @@ def main():
     a = ap.parse_args()
     n = a.n
     dev = BornDevice(0)
-    dev.set_tuning(a.ty, 2048)
+    tbs = [int(v) for v in a.tb.split(",")] if a.ty else [2048]
+    dev.set_tuning(a.ty, tbs[0])
     dev.create_space((12.5, 12.5, 12.5), 32.0, 0.001, (n, n, n))
     dev.set_velocity(np.full((n, n, n), (3000 * 0.001) ** 2, np.float32))
     out = {"n": n, "reps": a.reps}
@@ -56,14 +61,18 @@ def main():
             dev.set_x_per_lane(xl)
             for pf in [int(v) for v in a.prefetch.split(",")]:
                 dev.set_prefetch(pf)
-                for name, form, act, fmt, B in VARIANTS:
-                    if a.variants != "all" and name not in a.variants.split(","):
-                        continue
-                    ms = dev.time_step(form, act, fmt, a.reps)
-                    key = (name + (f"_x{xl}" if xl != 1 else "") +
-                           {1: "_ring", 0: "_noring", -1: ""}[ring] + (f"_pf{pf}" if pf else ""))
-                    out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
-                                "gpts": n ** 3 / ms / 1e6}
+                for tb in tbs:
+                    dev.set_tuning(a.ty, tb)
+                    for name, form, act, fmt, B in VARIANTS:
+                        if a.variants != "all" and name not in a.variants.split(","):
+                            continue
+                        ms = dev.time_step(form, act, fmt, a.reps)
+                        key = (name + (f"_x{xl}" if xl != 1 else "") +
+                               {1: "_ring", 0: "_noring", -1: ""}[ring] +
+                               (f"_pf{pf}" if pf else "") +
+                               (f"_ty{a.ty}_tb{tb}" if a.ty else ""))
+                        out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
+                                    "gpts": n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()
 
