@@ -41,6 +41,63 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+CALLER = r"""
+#include "gpu_funcs_3d.h"
+#include "born_rtm.h"
+#include "born_op.h"
+#include "born_solver.h"
+#include "born_unet.h"
+
+/* Every declared function, by address: the link fails on a missing one. */
+static void (*const every_symbol[])(void) = {
+%s
+};
+
+/* A host driver in the order the reference's rtm_zero_op calls its device
+   layer (set-up, shot transfers, source propagation, imaging); it is only
+   compiled and linked, never run. */
+int drive(int n1, int n2, int n3, int nt, float *vel, float **sinc, int *slocs, float *src,
+          int nx, int ny, int *rlocs, float *data, float *img, float *p0, float *p1) {
+  setup_cuda(1, 0, 0);
+  create_gpu_space(10.f, 10.f, 10.f, 30.f, 0.001f, 0.001f, n1, n2, n3);
+  transfer_vel_func1(n1, n2, n3, vel);
+  transfer_sinc_table_s(8, 1, sinc);
+  transfer_sinc_table_d(8, 1, sinc);
+  transfer_source_func(9, nt + 8, slocs, src);
+  source_prop(n1, n2, n3, 0, 1, p0, p1, 1, 9, nt);
+  transfer_receiver_func(nx, ny, nt + 8, rlocs, data);
+  rtm_adjoint(n1, n2, n3, 1, p0, p1, img, 9, nt);
+  rtm_forward(n1, n2, n3, 1, img, data, 9, nt, nt + 8, nx, ny);
+  return (int)(sizeof every_symbol / sizeof every_symbol[0]);
+}
+"""
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_headers_compile_and_link_as_c_and_cpp(tmp_path, lang):
+    """Every header compiles as C and as C++ in one translation unit, and a
+    caller that takes the address of every declared function and drives
+    the reference-signature layer links against the library."""
+    import shutil
+    import subprocess
+    cc = {"c": "gcc", "c++": "g++"}[lang]
+    if shutil.which(cc) is None:
+        pytest.skip(f"{cc} not installed")
+    _lib.load()  # builds the library when it is missing
+    names = sorted({n for _, n in declared_functions()})
+    refs = ",\n".join(f"  (void (*)(void)){n}" for n in names)
+    src = tmp_path / ("caller.c" if lang == "c" else "caller.cpp")
+    src.write_text(CALLER % refs + "int main(void) { return 0; }\n")
+    libdir = os.path.join(ROOT, "sep_code_amd", "lib")
+    cmd = [cc, "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+           "-o", str(tmp_path / "caller"), "-L", libdir, "-lborn_rtm",
+           "-Wl,--allow-shlib-undefined", "-Wl,-rpath," + libdir]
+    if lang == "c":
+        cmd.insert(1, "-std=c11")
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
 def test_python_binding_covers_born_rtm_header():
     decl = {n for h, n in declared_functions() if h == "born_rtm.h"}
     assert decl <= set(_lib.SIGNATURES), decl - set(_lib.SIGNATURES)
