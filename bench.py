@@ -80,6 +80,24 @@ def rtm_shot(dev, w):
     dev.rtm_adjoint(w.jt, w.nt, w.jimg)
 
 
+def cells_per_lane(n1, want):
+    """The forward-form kernel the library runs for this n1: cells per lane
+    asked for (1, 2, 4) where n1 allows it, or automatically (0) the widest
+    whose tiles keep at least 3/4 of the lanes busy."""
+    if want == 0:
+        want = 1
+        for k in (4, 2):
+            w = 64 * k
+            if n1 % k == 0 and 4 * n1 >= 3 * (-(-n1 // w) * w):
+                want = k
+                break
+    if want == 4 and n1 % 4 == 0:
+        return 4
+    if want >= 2 and n1 % 2 == 0:
+        return 2
+    return 1
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -248,11 +266,10 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": {0: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)",
-                       1: "brtm::k_step (VL, fused sponge)",
+            "kernel": {1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
                        4: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)"
-                       }[a.x_per_lane],
+                       }[cells_per_lane(w.n[0], a.x_per_lane)],
             "kernel_ms": round(k_ms, 4), "launches": kcount,
             "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
     cpu = None
