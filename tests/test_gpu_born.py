@@ -146,6 +146,29 @@ def make_shot(dev, n=(48, 44, 40), nt=120, jt=1, jimg=1, seed=3, cimg=1e-3, bc_a
     return P, r
 
 
+@pytest.mark.parametrize("n", [(64, 40, 36), (260, 20, 24)])
+def test_two_slice_lookahead_is_bit_identical(dev, n):
+    """The load lookahead changes only when loads are issued, not the
+    arithmetic: the plain ring step, and the source propagation with the
+    source injected inside the step, give the same bits with one and with
+    two slices of lookahead (short z chunks included)."""
+    announce(f"lookahead bit-exact n={n}")
+    P, r = make_shot(dev, n=n, nt=80)
+    p1, p0 = random_field(r, n), random_field(r, n)
+    out = {}
+    for pf in (1, 2):
+        set_variant(dev, (4, 1, pf))
+        try:
+            a, b = dev.propagate(0, 30, p1, p0)
+            s0, s1 = dev.source_prop(1, P.nt, 1, store="none", get_last=True)
+        finally:
+            set_variant(dev, (0, -1))
+        out[pf] = (a, b, s0, s1)
+    assert np.abs(out[1][2]).max() > 0
+    for x, y in zip(out[1], out[2]):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 3)])
 def test_source_prop_parity(dev, jt, jimg):
     announce(f"source_prop parity jt={jt} jimg={jimg}")
