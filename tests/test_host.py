@@ -169,3 +169,16 @@ def test_lsrtm_workload_shape():
     assert len(w.shots) == 8 and w.vel.shape == (64, 64, 64)
     assert G.cfl_number(w.vel.max(), w.dt, w.d) < 0.4529
     assert np.count_nonzero(w.refl) > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with status 2 and says why,
+    before touching any device."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
