@@ -231,9 +231,9 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the whole shot domain "
+        print(json.dumps({"error": "bench.py runs on one GPU: the whole shot domain "
                                    "sits in one MI355X's HBM; multi-GPU sharding is by shot "
-                                   "and is not part of this build", "n_gpus": a.gpus}))
+                                   "and is not implemented here", "n_gpus": a.gpus}))
         sys.exit(2)
 
     from sep_code_amd import configs

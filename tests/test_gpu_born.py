@@ -1,6 +1,6 @@
 """GPU parity of the HIP Born path against the FP64 oracle (through the C ABI).
 
-Tolerances (stated by the north star): wavefields within 1e-5 relative L2 of
+Tolerances: wavefields within 1e-5 relative L2 of
 the FP64 oracle; images and adjoint dot-product tests within 1e-4."""
 import sys
 
