@@ -669,26 +669,45 @@ k_step_x2(const StepParams P) {
 // ---------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int FMT>
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Streaming loads: data a step reads exactly once and no other workgroup
+// touches (p_old, V, the snapshot, the image or model).  The nontemporal
+// hint keeps these lines from displacing, in L2, the field lines that the
+// neighbouring tiles are about to read as their halos.
+__device__ __forceinline__ f32x4 ld4_stream(const float *a, int i) {
+  return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + i));
+}
+
+// Four consecutive snapshot values decoded to FP32; STREAM selects the
+// streaming (nontemporal) load.
+template <int FMT, bool STREAM = false>
 __device__ __forceinline__ f32x4 snap4_at(const void *s, const float *sscale, int i, int z) {
   f32x4 r = {0.f, 0.f, 0.f, 0.f};
   if constexpr (FMT == FMT_F32) {
-    r = *reinterpret_cast<const f32x4 *>(static_cast<const float *>(s) + i);
-  } else if constexpr (FMT == FMT_BF16) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(s) + i);
+    const f32x4 *a = reinterpret_cast<const f32x4 *>(static_cast<const float *>(s) + i);
+    if constexpr (STREAM) r = __builtin_nontemporal_load(a);
+    else r = *a;
+  } else {
+    // 16-bit codes: four of them in 8 bytes
+    const u32x2 *a = reinterpret_cast<const u32x2 *>(static_cast<const uint16_t *>(s) + i);
+    u32x2 u;
+    if constexpr (STREAM) u = __builtin_nontemporal_load(a);
+    else u = *a;
     const float sc = sscale[z];
-    r.x = __uint_as_float(u.x << 16) * sc;
-    r.y = __uint_as_float(u.x & 0xffff0000u) * sc;
-    r.z = __uint_as_float(u.y << 16) * sc;
-    r.w = __uint_as_float(u.y & 0xffff0000u) * sc;
-  } else if constexpr (FMT == FMT_F16) {
-    const __half2 *h = reinterpret_cast<const __half2 *>(static_cast<const __half *>(s) + i);
-    const float2 a = __half22float2(h[0]), b = __half22float2(h[1]);
-    const float sc = sscale[z];
-    r.x = a.x * sc;
-    r.y = a.y * sc;
-    r.z = b.x * sc;
-    r.w = b.y * sc;
+    if constexpr (FMT == FMT_BF16) {
+      r.x = __uint_as_float(u.x << 16) * sc;
+      r.y = __uint_as_float(u.x & 0xffff0000u) * sc;
+      r.z = __uint_as_float(u.y << 16) * sc;
+      r.w = __uint_as_float(u.y & 0xffff0000u) * sc;
+    } else if constexpr (FMT == FMT_F16) {
+      const float2 lo = __half22float2(__builtin_bit_cast(__half2, (unsigned int)u.x));
+      const float2 hi = __half22float2(__builtin_bit_cast(__half2, (unsigned int)u.y));
+      r.x = lo.x * sc;
+      r.y = lo.y * sc;
+      r.z = hi.x * sc;
+      r.w = hi.y * sc;
+    }
   }
   return r;
 }
@@ -984,12 +1003,12 @@ k_step_x4r(const StepParams P) {
       const int y = yb + r;
       const bool ok = xmem && y < n2;
       const int i = z * s3 + y * n1 + x;
-      opo[r] = ld4(P.po, i, ok);
-      oV[r] = ld4(P.V, i, ok);
+      const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+      opo[r] = ok ? ld4_stream(P.po, i) : zero;
+      oV[r] = ok ? ld4_stream(P.V, i) : zero;
       if constexpr (ACT != ACT_NONE) {
-        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-        oS[r] = ok ? snap4_at<FMT>(P.snap, P.sscale, i, z) : zero;
-        oX[r] = ld4(ACT == ACT_IMG ? P.img : P.m, i, ok);
+        oS[r] = ok ? snap4_at<FMT, true>(P.snap, P.sscale, i, z) : zero;
+        oX[r] = ok ? ld4_stream(ACT == ACT_IMG ? P.img : P.m, i) : zero;
       }
     }
   };
