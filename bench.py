@@ -42,7 +42,7 @@ BYTES_PER_CELL = 16.0
 # HBM bytes per launch of the plain VL step (k_step_x4r, 512^3) from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration (2.0 / 1.0):
 # profiles/stencil512_traffic.md.  Used for the default configuration only.
-TRAFFIC_X4R_512 = (2.0 * 798749 + 508032) * 1024.0
+TRAFFIC_X4R_512 = (2.0 * 794832 + 508032) * 1024.0
 
 
 def log(msg):

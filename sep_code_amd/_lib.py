@@ -67,6 +67,7 @@ SIGNATURES = {
     "brtm_set_halo_ring": [_P, _I],
     "brtm_set_receiver_fusion": [_P, _I],
     "brtm_set_prefetch": [_P, _I],
+    "brtm_set_stream_loads": [_P, _I],
     "brtm_set_born_path": [_P, _I],
     "brtm_kernel_timing": [_P, _I, _I, _I, _I],
     "brtm_kernel_timing_result": [_P, C.POINTER(_D), C.POINTER(_I)],

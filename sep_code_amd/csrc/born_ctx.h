@@ -83,6 +83,7 @@ struct brtm_ctx {
   int halo_ring = -1;  // six-plane LDS ring: -1 automatic (on for 4 cells per lane), 0, 1
   bool rec_fusion = true;  // receivers inside the four-cell ring step where they allow it
   int prefetch = 0;        // load lookahead of the plain four-cell ring step: 0 auto, 1, 2
+  int stream_loads = -1;   // streaming loads in the four-cell ring step: -1 auto, 0, 1
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)

@@ -49,6 +49,9 @@ struct StepParams {
   // slices of global-load lookahead of the four-cell ring step (1, or 2 for
   // the plain step)
   int pf;
+  // 1: the four-cell ring step reads p_old, V, the snapshot and the
+  // image/model with nontemporal (streaming) loads
+  int stream;
   const int *rmap;
   float *rtr;
   const float *rscale;

@@ -909,7 +909,7 @@ k_step_x4(const StepParams P) {
 // 149 KB of LDS, which the one workgroup per CU that the registers allow
 // can have.
 // ---------------------------------------------------------------------------
-template <int TY, int RPT, int FMT, int ACT, int REC = REC_NONE, int PF = 1>
+template <int TY, int RPT, int FMT, int ACT, int REC = REC_NONE, int PF = 1, bool STREAM = false>
 __global__ void __launch_bounds__(64 * (TY / RPT))
 k_step_x4r(const StepParams P) {
   static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
@@ -953,6 +953,15 @@ k_step_x4r(const StepParams P) {
   auto ld4 = [&](const float *a, int i, bool ok) -> f32x4 {
     f32x4 v = {0.f, 0.f, 0.f, 0.f};
     if (ok) v = *reinterpret_cast<const f32x4 *>(a + i);
+    return v;
+  };
+  // the lane's own streams (read once, by this tile only)
+  auto ldc = [&](const float *a, int i, bool ok) -> f32x4 {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+      if constexpr (STREAM) v = ld4_stream(a, i);
+      else v = *reinterpret_cast<const f32x4 *>(a + i);
+    }
     return v;
   };
 
@@ -1003,12 +1012,12 @@ k_step_x4r(const StepParams P) {
       const int y = yb + r;
       const bool ok = xmem && y < n2;
       const int i = z * s3 + y * n1 + x;
-      const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-      opo[r] = ok ? ld4_stream(P.po, i) : zero;
-      oV[r] = ok ? ld4_stream(P.V, i) : zero;
+      opo[r] = ldc(P.po, i, ok);
+      oV[r] = ldc(P.V, i, ok);
       if constexpr (ACT != ACT_NONE) {
-        oS[r] = ok ? snap4_at<FMT, true>(P.snap, P.sscale, i, z) : zero;
-        oX[r] = ok ? ld4_stream(ACT == ACT_IMG ? P.img : P.m, i) : zero;
+        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+        oS[r] = ok ? snap4_at<FMT, STREAM>(P.snap, P.sscale, i, z) : zero;
+        oX[r] = ldc(ACT == ACT_IMG ? P.img : P.m, i, ok);
       }
     }
   };
@@ -1565,36 +1574,44 @@ hipError_t launch_ring_t(int act, int fmt, const StepParams &P, hipStream_t st) 
   return hipErrorInvalidValue;
 }
 
+// The ring step with its receiver mode, lookahead and load policy.
+template <int TY, int RPT, int FMT, int ACT, bool STREAM>
+hipError_t launch_x4r(const StepParams &P, hipStream_t st, int rec, dim3 grid, dim3 block) {
+  if constexpr (ACT == ACT_NONE) {
+    if (P.pf == 2 && rec == REC_NONE) {
+      k_step_x4r<TY, RPT, FMT, ACT, REC_NONE, 2, STREAM><<<grid, block, 0, st>>>(P);
+      return hipGetLastError();
+    }
+    if (P.pf == 2 && rec == REC_INJECT) {
+      k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT, 2, STREAM><<<grid, block, 0, st>>>(P);
+      return hipGetLastError();
+    }
+  }
+  if (rec == REC_NONE) {
+    k_step_x4r<TY, RPT, FMT, ACT, REC_NONE, 1, STREAM><<<grid, block, 0, st>>>(P);
+  } else if (rec == REC_INJECT) {
+    if constexpr (ACT != ACT_BORN)
+      k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT, 1, STREAM><<<grid, block, 0, st>>>(P);
+    else
+      return hipErrorInvalidValue;
+  } else if (rec == REC_EXTRACT) {
+    if constexpr (ACT != ACT_IMG)
+      k_step_x4r<TY, RPT, FMT, ACT, REC_EXTRACT, 1, STREAM><<<grid, block, 0, st>>>(P);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <bool RING, int TY, int RPT, int FMT, int ACT>
 hipError_t launch_x4_kernel(const StepParams &P, hipStream_t st, int rec) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.nty * P.ntz);
   if constexpr (RING) {
-    if constexpr (ACT == ACT_NONE) {
-      if (P.pf == 2 && rec == REC_NONE) {
-        k_step_x4r<TY, RPT, FMT, ACT, REC_NONE, 2><<<grid, block, 0, st>>>(P);
-        return hipGetLastError();
-      }
-      if (P.pf == 2 && rec == REC_INJECT) {
-        k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT, 2><<<grid, block, 0, st>>>(P);
-        return hipGetLastError();
-      }
-    }
-    if (rec == REC_NONE) {
-      k_step_x4r<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);
-    } else if (rec == REC_INJECT) {
-      if constexpr (ACT != ACT_BORN)
-        k_step_x4r<TY, RPT, FMT, ACT, REC_INJECT><<<grid, block, 0, st>>>(P);
-      else
-        return hipErrorInvalidValue;
-    } else if (rec == REC_EXTRACT) {
-      if constexpr (ACT != ACT_IMG)
-        k_step_x4r<TY, RPT, FMT, ACT, REC_EXTRACT><<<grid, block, 0, st>>>(P);
-      else
-        return hipErrorInvalidValue;
-    } else {
-      return hipErrorInvalidValue;
-    }
+    return P.stream ? launch_x4r<TY, RPT, FMT, ACT, true>(P, st, rec, grid, block)
+                    : launch_x4r<TY, RPT, FMT, ACT, false>(P, st, rec, grid, block);
   } else {
     if (rec != REC_NONE) return hipErrorInvalidValue;
     k_step_x4<TY, RPT, FMT, ACT><<<grid, block, 0, st>>>(P);

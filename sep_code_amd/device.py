@@ -194,6 +194,12 @@ class BornDevice:
         0 automatic (the default)."""
         call("brtm_set_prefetch", self.h, int(slices))
 
+    def set_stream_loads(self, mode):
+        """Nontemporal loads of the data the four-cell ring step reads once:
+        1 on, 0 off, -1 automatic (the default: on when a step moves more
+        than 512 MB)."""
+        call("brtm_set_stream_loads", self.h, int(mode))
+
     def set_receiver_fusion(self, on):
         """Receivers injected / extracted inside the four-cell ring step when
         they share one plane (1, the default) or by separate kernels (0)."""

@@ -13,7 +13,8 @@ cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 "_x2" / "_x4" suffix), halo-ring setting in --ring (1 on, suffix "_ring";
 0 off, suffix "_noring"; -1 the library's automatic choice, no suffix) and
 load lookahead in --prefetch (1 or 2 slices, suffix "_pf1" / "_pf2"; 0
-automatic, no suffix).  With --ty, every workgroup target in --tb is timed
+automatic, no suffix), streaming loads in --stream (1 on, suffix "_stream";
+0 off, suffix "_nostream"; -1 automatic, no suffix).  With --ty, every workgroup target in --tb is timed
 (suffix "_ty<rows>_tb<target>").
 """
 from __future__ import annotations
@@ -42,6 +43,8 @@ def main():
     ap.add_argument("--variants", default="all")
     ap.add_argument("--xl", default="1,2")
     ap.add_argument("--ring", default="-1", help="comma list of halo-ring settings (-1, 0, 1)")
+    ap.add_argument("--stream", default="-1",
+                    help="comma list of streaming-load settings of the ring step (-1 auto, 0, 1)")
     ap.add_argument("--prefetch", default="0",
                     help="comma list of load lookaheads of the plain ring step (0 auto, 1, 2)")
     a = ap.parse_args()
@@ -59,8 +62,10 @@ def main():
         dev.set_halo_ring(ring)
         for xl in [int(v) for v in a.xl.split(",")]:
             dev.set_x_per_lane(xl)
-            for pf in [int(v) for v in a.prefetch.split(",")]:
+            for pf, sl in [(int(v), int(u)) for v in a.prefetch.split(",")
+                           for u in a.stream.split(",")]:
                 dev.set_prefetch(pf)
+                dev.set_stream_loads(sl)
                 for tb in tbs:
                     dev.set_tuning(a.ty, tb)
                     for name, form, act, fmt, B in VARIANTS:
@@ -70,6 +75,7 @@ def main():
                         key = (name + (f"_x{xl}" if xl != 1 else "") +
                                {1: "_ring", 0: "_noring", -1: ""}[ring] +
                                (f"_pf{pf}" if pf else "") +
+                               ({-1: "", 0: "_nostream", 1: "_stream"}[sl]) +
                                (f"_ty{a.ty}_tb{tb}" if a.ty else ""))
                         out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
                                     "gpts": n ** 3 / ms / 1e6}

@@ -169,6 +169,40 @@ def test_two_slice_lookahead_is_bit_identical(dev, n):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("store", ["f32", "bf16", "f16"])
+def test_stream_loads_are_bit_identical(dev, store):
+    """Streaming (nontemporal) loads change only the cache policy: the plain
+    ring step, Born modelling (fused Born source, receivers extracted in the
+    step) and the adjoint (fused imaging, receivers injected in the step)
+    give the same bits with the hint on and off."""
+    announce(f"stream loads bit-exact store={store}")
+    n = (64, 40, 36)
+    P, r = make_shot(dev, n=n, nt=60, jimg=2)
+    p1, p0 = random_field(r, n), random_field(r, n)
+    m = random_field(r, n)
+    d = r.standard_normal((len(P.rlocs), P.ntr_d))
+    out = {}
+    for mode in (0, 1):
+        set_variant(dev, (4, 1))
+        dev.set_stream_loads(mode)
+        try:
+            a, b = dev.propagate(0, 30, p1, p0)
+            dev.source_prop(1, P.nt, 2, store=store)
+            dev.set_model(m)
+            dev.rtm_forward(1, P.nt, 2)
+            D = dev.get_receivers()
+            dev.set_receivers(P.rlocs, d)
+            dev.zero_image()
+            dev.rtm_adjoint(1, P.nt, 2)
+            out[mode] = (a, b, D, dev.get_image())
+        finally:
+            dev.set_stream_loads(-1)
+            set_variant(dev, (0, -1))
+    assert np.abs(out[0][3]).max() > 0 and np.abs(out[0][2]).max() > 0
+    for x, y in zip(out[0], out[1]):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 3)])
 def test_source_prop_parity(dev, jt, jimg):
     announce(f"source_prop parity jt={jt} jimg={jimg}")
