@@ -11,7 +11,8 @@ value = RTM Gpts/s = 2 * n1*n2*n3 * nt / time per shot (the two wavefields
 an RTM must advance; the checkpoint recompute is overhead, not credited).
 
 roofline: the dominant kernel is the fused VL stencil step (k_step_x4r at
-512^3: four cells per lane on the six-plane halo ring, no imaging; it runs
+512^3: four cells per lane on the six-plane halo ring, streaming loads of
+p_old and V, no imaging; it runs
 for the source propagation and the recompute).  Its launches inside the timed region are bracketed by
 HIP events on the library stream; achieved = 16 B per cell (read p_n,
 p_{n-1}, V; write p_{n+1}) * n1*n2*n3 / mean launch duration.  traffic is
@@ -43,6 +44,9 @@ BYTES_PER_CELL = 16.0
 # FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration (2.0 / 1.0):
 # profiles/stencil512_traffic.md.  Used for the default configuration only.
 TRAFFIC_X4R_512 = (2.0 * 794832 + 508032) * 1024.0
+# The library's ring step reads the data it uses once with nontemporal loads
+# when a step moves more than this many bytes (16 per cell).
+STREAM_MIN_STEP_BYTES = 512 * 2 ** 20
 
 
 def log(msg):
@@ -279,7 +283,9 @@ def main():
             "kernel": {1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
                        4: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)"
-                       }[cells_per_lane(w.n[0], a.x_per_lane)],
+                       }[cells_per_lane(w.n[0], a.x_per_lane)] + (
+                           " + streaming loads" if cells_per_lane(w.n[0], a.x_per_lane) == 4
+                           and 16 * ncell > STREAM_MIN_STEP_BYTES else ""),
             "kernel_ms": round(k_ms, 4), "launches": kcount,
             "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
     cpu = None
