@@ -102,6 +102,22 @@ def cells_per_lane(n1, want):
     return 1
 
 
+def plain_step_kernel(n, want):
+    """(cells per lane, tile rows) of the plain forward step the library
+    runs on grid n = (n1, n2, n3) with automatic tiles: the default shape,
+    or 64-wide 8-row tiles when that shape would leave some of MI355X's 256
+    compute units without a workgroup."""
+    xl = cells_per_lane(n[0], want)
+    ty, tb = {4: (16, 256), 2: (16, 2048), 1: (32, 1024)}[xl]
+    if want == 0:
+        tiles = -(-n[0] // (64 * xl)) * -(-n[1] // ty)
+        nz = n[2] - 8
+        zc = min(max(-(-tiles * nz // tb), 8), nz)
+        if tiles * -(-nz // zc) < 256:
+            return 1, 8
+    return xl, ty
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -277,15 +293,17 @@ def main():
     default_cfg = w.n == (512, 512, 512) and a.x_per_lane in (0, 4) and a.ty == 0
     if traffic is None and default_cfg:
         traffic = TRAFFIC_X4R_512
+    kxl, kty = plain_step_kernel(w.n, a.x_per_lane) if a.ty == 0 else (
+        cells_per_lane(w.n[0], a.x_per_lane), a.ty)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel": {1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
                        4: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)"
-                       }[cells_per_lane(w.n[0], a.x_per_lane)] + (
-                           " + streaming loads" if cells_per_lane(w.n[0], a.x_per_lane) == 4
-                           and 16 * ncell > STREAM_MIN_STEP_BYTES else ""),
+                       }[kxl] + f", {kty}-row tiles" + (
+                           " + streaming loads" if kxl == 4 and 16 * ncell > STREAM_MIN_STEP_BYTES
+                           else ""),
             "kernel_ms": round(k_ms, 4), "launches": kcount,
             "stencil_gpts": round(ncell / (k_ms * 1e-3) / 1e9, 2)}
     cpu = None

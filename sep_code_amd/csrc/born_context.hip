@@ -130,7 +130,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     else if (want >= 2 && c->n1 % 2 == 0) xl = 2;
   }
   // halo ring: automatic (-1) means on for the 4-cell kernel
-  const bool ring = c->halo_ring < 0 ? xl == 4 : c->halo_ring != 0;
+  bool ring = c->halo_ring < 0 ? xl == 4 : c->halo_ring != 0;
   if (ty == 0) {
     if (xl == 4) { ty = (ring && act == ACT_IMG && fmt == FMT_F32) ? 8 : 16; tb = 256; }
     else if (xl == 2) { ty = 16; tb = 2048; }
@@ -140,6 +140,17 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   }
   if (xl == 4 && ty == 32) ty = 16;  // the 4-cell kernel's tiles are 8 or 16 rows
   plan_step(P, ty, tb, 64 * xl);
+  // Small grids: when the automatic shape leaves compute units without a
+  // workgroup, 64-wide 8-row tiles give several times as many (128^3: 480
+  // workgroups instead of 120, the plain step 10.7 us instead of 12.8).
+  if (c->ty_tile == 0 && c->x_per_lane == 0 && c->halo_ring < 0 &&
+      (long)P.ntx * P.nty * P.ntz < c->ncu) {
+    xl = 1;
+    ring = false;
+    ty = 8;
+    tb = 2048;
+    plan_step(P, ty, tb, 64);
+  }
   const bool timed =
       c->ktime_on && form == c->ktime_form && act == c->ktime_act && fmt == c->ktime_fmt;
   if (timed) {
@@ -532,6 +543,10 @@ int brtm_create(int device, brtm_ctx **out) {
   BRTM_HIP(hipSetDevice(device));
   brtm_ctx *c = new brtm_ctx();
   c->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      ncu > 0)
+    c->ncu = ncu;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;

@@ -68,6 +68,7 @@ struct Store {
 
 struct brtm_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   bool have_space = false, have_vel = false;
   int n1 = 0, n2 = 0, n3 = 0;
