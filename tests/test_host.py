@@ -162,6 +162,12 @@ def test_library_sinc_table_matches_oracle():
     from sep_code_amd.operator import sinc_table
     for ntab in (1, 3, 4):
         np.testing.assert_allclose(sinc_table(ntab), O.sinc_table(ntab), atol=1e-6)
+    # and against the reference's own sinc_bob output (tests/golden/ref_outputs.json)
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "ref_outputs.json")) as f:
+        ref = json.load(f)["sinc"]
+    for ntab, t in ref.items():
+        np.testing.assert_allclose(sinc_table(int(ntab)), np.array(t), atol=1e-4)
 
 
 def test_lsrtm_workload_shape():

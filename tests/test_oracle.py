@@ -1,7 +1,9 @@
 """Pins of the CPU oracle (oracle/) -- each against something other than the
 oracle's own formula: closed forms, library routines, the reference's own code
 built from its sources (oracle/_ref), adjoint identities or brute force."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -349,6 +351,28 @@ def test_sinc_matches_reference_sinc_bob(ntab):
     out = np.zeros((ntab, 8), np.float32)
     ref.ref_sinc_table(ntab, 8, C.c_void_p(out.ctypes.data))
     np.testing.assert_allclose(O.sinc_table(ntab), out, atol=1e-4)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_outputs.json")
+
+
+def test_oracle_matches_reference_fixture():
+    """Against tests/golden/ref_outputs.json, outputs of the reference's own
+    Born/sinc_bob.cpp and Born/cgstep.cpp on seeded inputs (written by
+    tests/golden/make_ref_fixtures.py from oracle/_ref): runs where the
+    reference sources are absent.  Sinc tables for 1-8 phases to 1e-4; cgstep
+    models and residuals to float precision."""
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    assert set(g["sinc"]) == {"1", "2", "3", "4", "5", "8"}
+    for ntab, t in g["sinc"].items():
+        np.testing.assert_allclose(O.sinc_table(int(ntab)), np.array(t), atol=1e-4)
+    assert len(g["cgstep"]) == 3
+    for c in g["cgstep"]:
+        A, d = np.array(c["A"], np.float64), np.array(c["d"], np.float64)
+        m64, res64 = O.cgstep_solve(A, d, c["niter"])
+        np.testing.assert_allclose(np.array(c["m"]), m64, rtol=2e-4, atol=2e-5)
+        np.testing.assert_allclose(np.array(c["resid"]), res64, rtol=2e-4)
 
 
 # --- A24/A25 CG --------------------------------------------------------------
