@@ -198,11 +198,11 @@ int brtm_set_receiver_fusion(brtm_ctx *ctx, int on);
  * MI355X).  No counterpart in the reference; the same results. */
 int brtm_set_prefetch(brtm_ctx *ctx, int slices);
 
-/* Streaming (nontemporal) loads of the data the four-cell ring step reads
- * once (p_old, V, the snapshot, the image or model): 1 on, 0 off, -1 (the
+/* Streaming (nontemporal) loads of p_old, V, 16-bit snapshots and the
+ * model, which the four-cell ring step reads once: 1 on, 0 off, -1 (the
  * default) on when a step moves more than 512 MB, twice the Infinity Cache
- * (512^3 yes, 256^3 no).  No counterpart in the reference; the same
- * results. */
+ * (512^3 yes, 256^3 no).  The image and FP32 snapshots stream at every
+ * size.  No counterpart in the reference; the same results. */
 int brtm_set_stream_loads(brtm_ctx *ctx, int mode);
 
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes

@@ -195,9 +195,10 @@ class BornDevice:
         call("brtm_set_prefetch", self.h, int(slices))
 
     def set_stream_loads(self, mode):
-        """Nontemporal loads of the data the four-cell ring step reads once:
-        1 on, 0 off, -1 automatic (the default: on when a step moves more
-        than 512 MB)."""
+        """Nontemporal loads of p_old, V, 16-bit snapshots and the model in
+        the four-cell ring step: 1 on, 0 off, -1 automatic (the default: on
+        when a step moves more than 512 MB).  The image and FP32 snapshots
+        stream at every size."""
         call("brtm_set_stream_loads", self.h, int(mode))
 
     def set_receiver_fusion(self, on):
