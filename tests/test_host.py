@@ -188,3 +188,20 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
+
+
+def test_bench_names_the_kernel_the_library_runs():
+    """bench.py's roofline label mirrors the library's automatic kernel
+    choice: four cells per lane with 16-row tiles at 512^3 and 256^3, the
+    1-cell kernel with 8-row tiles where the automatic shape would leave
+    MI355X's 256 compute units without a workgroup (128^3), and the forced
+    cells per lane where n1 allows them."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.plain_step_kernel((512, 512, 512), 0) == (4, 16)
+    assert bench.plain_step_kernel((256, 256, 256), 0) == (4, 16)
+    assert bench.plain_step_kernel((128, 128, 128), 0) == (1, 8)
+    assert bench.plain_step_kernel((296, 296, 300), 0) == (2, 16)
+    assert bench.plain_step_kernel((512, 512, 512), 2) == (2, 16)
+    assert bench.cells_per_lane(70, 4) == 2 and bench.cells_per_lane(71, 4) == 1
