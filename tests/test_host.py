@@ -205,3 +205,24 @@ def test_bench_names_the_kernel_the_library_runs():
     assert bench.plain_step_kernel((296, 296, 300), 0) == (2, 16)
     assert bench.plain_step_kernel((512, 512, 512), 2) == (2, 16)
     assert bench.cells_per_lane(70, 4) == 2 and bench.cells_per_lane(71, 4) == 1
+
+
+def test_bench_cpu_full_shot_matches_fp64_oracle():
+    """The whole-shot CPU baseline (bench.cpu_full_shot, the FP32 OpenMP
+    restatement with denormals flushed) images the same shot as the FP64
+    oracle to 1e-5, for both of the workload's source stores."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    w = configs.rtm128(nt=40)
+    sh = w.shots[0]
+    D = np.random.default_rng(5).standard_normal((len(sh.rec_locs), w.ntr)).astype(np.float32)
+    P = O.Problem(w.n, w.d, w.V, w.bc_a, w.bc_b, src_locs=sh.src_locs, src_tr=sh.src_traces,
+                  rec_locs=sh.rec_locs, ntr_d=w.ntr, jt=w.jt, nt=w.nt, jimg=w.jimg, cimg=w.cimg)
+    ref = P.rtm_adjoint(D)
+    assert np.abs(ref).max() > 0
+    for store in ("f32", "ckpt"):
+        w.store, w.ckpt = store, 16
+        r = bench.cpu_full_shot(w, D, ref, runs=1)
+        assert r["image_rel_diff_vs_gpu"] < 1e-5, (store, r)
+        assert r["value"] > 0 and r["kind"] == "port"
