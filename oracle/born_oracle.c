@@ -33,7 +33,11 @@
  * explicit transpose of the VL step (dense matrices on tiny grids); the
  * undamped leapfrog against exact time reversal; inject/extract and
  * forward/adjoint against dot-product tests; a 1-D plane wave against the
- * discrete dispersion relation; and the FP32 build against the FP64 build.
+ * discrete dispersion relation; the FP32 build against the FP64 build; and
+ * the time indexing of source propagation, Born modelling and the adjoint
+ * (sample it/jt, sinc row it%jt, step before injection, the +4 trace offset,
+ * the Born source from s^{it+1}) against the closed-form solution of the
+ * V = 0 undamped recursion, p^n = sum_{m<n} (n-m) f(m).
  */
 #include <math.h>
 #include <stdint.h>

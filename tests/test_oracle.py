@@ -294,6 +294,152 @@ def test_fp32_denormal_flush_is_invisible_and_restored():
     assert np.linalg.norm(img1 - img0) <= 1e-6 * np.linalg.norm(img0)
 
 
+# --- time recursion: closed forms with V = 0 and no sponge ------------------
+# With V = 0 and w = 1 both step forms reduce to p' = 2p - p_old at every
+# cell, so every cell runs its own scalar recursion from zero fields.  A
+# forcing f(m) added after step m then gives p^n = sum_{m<n} (n-m) f(m), the
+# double cumulative sum of f; the adjoint's backward recursion forced after
+# each step gives lambda^n = sum_{m>=n-1} (m-n+2) h(m).  The forcings below
+# follow the reference's own indexing, not the oracle's code:
+#   * step it reads sample id = it/jt with sinc row ii = it%jt
+#     (Born/gpu_funcs_3d.cu:228-229, 398-400, 598-601);
+#   * the step runs first and the injection follows it (:247-250);
+#   * source trace sample t+4 holds the wavelet sample t
+#     (Born/source_func_3d.cpp:127), so with the zero-shift row (tap 3) the
+#     source enters one sample late;
+#   * Born modelling adds c s^{it+1} m to q^{it+1} when it % jt == 0 and
+#     extracts the traces from that q^{it+1} at samples id+k (:423-462);
+#   * the adjoint injects the data after the step and images the injected
+#     lambda^{it+1} when ii == 0 (:626-667).
+# The sinc rows for jt = 2 are the reference's own sinc_bob output
+# (tests/golden/ref_outputs.json).  A one-step shift of any of these indices,
+# inject-before-step, or a lost or extra step fails one of the tests.
+def _ref_sinc(jt):
+    with open(GOLDEN) as f:
+        return np.array(json.load(f)["sinc"][str(jt)], np.float64)
+
+
+def _zero_velocity_case(jt, nt, jimg, seed=11):
+    """A 14^3 grid with V = 0, no sponge, a 9-point source around (6.3, 6.6)
+    on plane 5, receivers on two source cells and one cell the source never
+    reaches."""
+    r = rng(seed)
+    n = (14, 14, 14)
+    nt_data = (nt - 1) // jt + 1
+    ntr = nt_data + 8
+    w = r.standard_normal(nt_data)
+    slocs, str_ = O.source_geometry(n[0], n[1], 6.0, 6.3, 6.6, w, ntr)
+    rl = np.array([slocs[4], slocs[0], 8 + 8 * 14 + 8 * 196], np.int32)
+    sinc = _ref_sinc(jt) if jt > 1 else None
+    P = O.Problem(n, (10.0, 10.0, 10.0), np.zeros((14, 14, 14)), 0.0, 0.0, src_locs=slocs,
+                  src_tr=str_, sinc_s=sinc, rec_locs=rl, ntr_d=ntr, sinc_d=sinc, jt=jt, nt=nt,
+                  jimg=jimg, cimg=0.37)
+    scale = np.array([math.exp(-(6.0 - 5) ** 2 - (6.3 - i1) ** 2 - (6.6 - i2) ** 2)
+                      for i2 in (5, 6, 7) for i1 in (5, 6, 7)])
+    S = sinc if sinc is not None else np.eye(8)[3][None, :]
+    return P, r, w, scale, S
+
+
+def _forcing(tr, S, jt, nt):
+    """The injected value after step it: sum_k S[it%jt][k] tr[it/jt + k]."""
+    return np.array([S[it % jt] @ tr[it // jt:it // jt + 8] for it in range(nt)])
+
+
+def _source_field(w, scale_j, S, jt, nt):
+    """s^{it+1}, it = 0..nt-1, at one source point of weight scale_j: the
+    double cumulative sum of the forcing built from trace[t+4] = w[t]."""
+    tr = np.zeros(len(w) + 8)
+    tr[4:4 + len(w)] = scale_j * w
+    return np.cumsum(np.cumsum(_forcing(tr, S, jt, nt)))
+
+
+def test_source_prop_one_sample_delay_jt1():
+    """jt = 1: s^n = scale * sum_{t<=n-2} (n-1-t) w[t], so s^1 = 0 and
+    s^2 = scale * w[0] at every source point (the +4 trace offset read
+    through tap 3 of the zero-shift row)."""
+    nt = 40
+    P, _, w, scale, S = _zero_velocity_case(1, nt, 1)
+    snaps, last2 = P.source_prop()
+    cw = np.cumsum(np.cumsum(w))
+    for j in (0, 4, 8):
+        got = snaps[:, P.slocs[j]]            # snaps[it] = s^{it+1}
+        want = np.concatenate([[0.0], scale[j] * cw[:nt - 1]])
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+        assert got[0] == 0.0 and np.isclose(got[1], scale[j] * w[0], rtol=1e-14)
+    # last two fields: s^{nt-1}, s^{nt}
+    np.testing.assert_allclose(last2[:, P.slocs[4]], snaps[-2:, P.slocs[4]], rtol=0, atol=0)
+    # nothing outside the nine source cells moves (V = 0)
+    other = np.ones(P.ncell, bool)
+    other[P.slocs] = False
+    assert np.all(snaps[:, other] == 0)
+
+
+@pytest.mark.parametrize("jt,jimg", [(2, 2), (2, 3)])
+def test_source_prop_closed_form_two_steps_per_sample(jt, jimg):
+    """jt = 2 with the reference's two-phase sinc rows: s^{it+1} is the double
+    cumulative sum of f(it) = S[it%2] . trace[it/2 : it/2+8]; snapshots are
+    s^{it+1} for it % jimg == 0."""
+    nt = 2 * 25 + 1
+    P, _, w, scale, S = _zero_velocity_case(jt, nt, jimg)
+    snaps, last2 = P.source_prop()
+    for j in (1, 4, 6):
+        want = _source_field(w, scale[j], S, jt, nt)
+        np.testing.assert_allclose(snaps[:, P.slocs[j]], want[::jimg], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(last2[:, P.slocs[j]], want[-2:], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 2), (2, 3)])
+@pytest.mark.parametrize("live", [False, True])
+def test_born_forward_closed_form(jt, jimg, live):
+    """q^{it+1} = 2q^it - q^{it-1} + c s^{it+1} m [it % jimg == 0], then
+    D[id+k] += S_d[ii][k] q^{it+1}: at a receiver on a source cell the trace
+    is the sinc-spread double cumulative sum of c m s^{it+1}; a receiver the
+    source never reaches records zero.  Holds with stored snapshots and with
+    the source propagated alongside."""
+    nt = 31 if jt == 1 else 2 * 15 + 1
+    P, r, w, scale, S = _zero_velocity_case(jt, nt, jimg)
+    m = r.standard_normal(P.ncell)
+    snaps = None if live else P.source_prop()[0]
+    D = P.born_forward(m, snaps)
+    for rec, j in ((0, 4), (1, 0)):
+        x = P.slocs[j]
+        s = _source_field(w, scale[j], S, jt, nt)             # s^{it+1}
+        g = np.where(np.arange(nt) % jimg == 0, P.cimg * m[x] * s, 0.0)
+        q = np.cumsum(np.cumsum(g))                           # q^{it+1}
+        want = np.zeros(P.ntr_d)
+        for it in range(nt):
+            want[it // jt:it // jt + 8] += S[it % jt] * q[it]
+        np.testing.assert_allclose(D[rec], want, rtol=1e-11, atol=1e-14)
+    assert np.all(D[2] == 0)
+
+
+@pytest.mark.parametrize("jt,jimg", [(1, 1), (2, 2), (2, 3)])
+def test_rtm_adjoint_closed_form(jt, jimg):
+    """lambda^{it+1} = 2 lambda^{it+2} - lambda^{it+3} + sum_k S_d[ii][k]
+    D[id+k] from zero fields at nt+1, nt+2, and img += c s^{it+1}
+    lambda^{it+1} for it % jimg == 0: at a receiver on a source cell the image
+    is c sum s^{it+1} lambda^{it+1}, with lambda the reversed double
+    cumulative sum of the injected data.  The checkpointed adjoint agrees."""
+    nt = 31 if jt == 1 else 2 * 15 + 1
+    P, r, w, scale, S = _zero_velocity_case(jt, nt, jimg)
+    D = r.standard_normal((3, P.ntr_d))
+    snaps, _ = P.source_prop()
+    img = P.rtm_adjoint(D, snaps)
+    for rec, j in ((0, 4), (1, 0)):
+        x = P.slocs[j]
+        h = _forcing(D[rec], S, jt, nt)                       # injected after step it
+        lam = np.cumsum(np.cumsum(h[::-1]))[::-1]             # lambda^{it+1}
+        s = _source_field(w, scale[j], S, jt, nt)
+        want = P.cimg * sum(s[it] * lam[it] for it in range(0, nt, jimg))
+        assert np.isclose(img[x], want, rtol=1e-11), (img[x], want)
+    # the receiver the source never reaches images nothing; nor do cells
+    # without a receiver
+    mask = np.ones(P.ncell, bool)
+    mask[P.slocs[[4, 0]]] = False
+    assert np.all(img[mask] == 0)
+    np.testing.assert_array_equal(P.rtm_adjoint(D, None, ckpt=4), img)
+
+
 # --- host-side operators ---------------------------------------------------
 def test_dz_dt_window_are_adjoint_pairs():
     r = rng(7)
