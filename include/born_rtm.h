@@ -60,8 +60,12 @@ const char *brtm_version(void);
 /* Replaces setup_cuda (Born/gpu_funcs_3d.cu:10-45): bind a context to one
  * GPU (no peer access: one GPU holds the domain). */
 int brtm_create(int device, brtm_ctx **out);
-/* Vectors, operators and networks created on a context (born_op.h,
- * born_unet.h) hold it: destroy them before the context. */
+/* Frees the context's device memory.  Vectors, operators, solvers and
+ * networks created on it (born_op.h, born_solver.h, born_unet.h) may be
+ * destroyed before or after it: each holds a reference to the context's
+ * host record (stream, events), which goes with the last of them.  Between
+ * brtm_destroy and their own destroy call, every other call on them returns
+ * BRTM_ESTATE.  Destroying a context twice is an error. */
 int brtm_destroy(brtm_ctx *ctx);
 
 /* Replaces create_gpu_space (Born/gpu_funcs_3d.cu:777-803): grid spacing,

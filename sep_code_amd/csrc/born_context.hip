@@ -244,6 +244,7 @@ static int extract_data(brtm_ctx *c, int it, int jt, const float *p, const float
 }
 
 static int check_ready(brtm_ctx *c, int jt, int nt, int jimg) {
+  BRTM_LIVE(c);
   BRTM_REQUIRE(c->have_space, BRTM_ESTATE, "brtm_create_space has not been called");
   BRTM_REQUIRE(c->have_vel, BRTM_ESTATE, "brtm_set_velocity has not been called");
   BRTM_REQUIRE(jt >= 1 && nt >= 1 && jimg >= 1, BRTM_EINVAL, "jt, nt and jimg must be >= 1");
@@ -301,8 +302,12 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
   };
   if (ck_direct) {
     // the steps write only the update set: the pairs' halos must be zero
-    // like the work buffers', and pair 0 is (s^0, s^-1) = 0
+    // like the work buffers', and pair 0 is (s^0, s^-1) = 0.  The work
+    // buffers are written out of place here without a full clear, so their
+    // halos are cleared too (other calls may have filled them).
     BRTM_HIP(hipMemsetAsync(ck_cur(0), 0, 2 * fbytes, c->stream));
+    BRTM_HIP(launch_zero_halo(wx, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+    BRTM_HIP(launch_zero_halo(wy, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
     if (!keep) {
       const size_t nf = S.data.bytes / fbytes;  // whole buffer, also beyond count pairs
       BRTM_HIP(launch_zero_halo(ck_cur(0) + 2 * c->ncell, c->ncell, (int)nf - 2, c->n1, c->n2,
@@ -522,6 +527,19 @@ int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out) {
   return BRTM_OK;
 }
 
+void ctx_retain(brtm_ctx *c) { c->refs.fetch_add(1); }
+
+void ctx_release(brtm_ctx *c) {
+  if (c->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : c->ktime_ev) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
 }  // namespace brtm
 
 using namespace brtm;
@@ -559,20 +577,19 @@ int brtm_create(int device, brtm_ctx **out) {
 
 int brtm_destroy(brtm_ctx *c) {
   if (!c) return BRTM_OK;
+  BRTM_REQUIRE(!c->destroyed, BRTM_ESTATE, "brtm_destroy called twice on one context");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &s : c->stores) {
-    s.release();
-  }
+  // The device memory goes now; objects still alive on the context keep
+  // only its host record (stream, events) until they are destroyed.
+  c->stores.clear();
   for (DevBuf *b : {&c->d_wtab, &c->V, &c->fa, &c->fb, &c->fc, &c->fd, &c->img, &c->mdl,
-                    &c->seg, &c->partials, &c->dotv, &c->slocs, &c->strc, &c->rlocs, &c->rmap, &c->smap,
-                    &c->traces})
+                    &c->seg, &c->mvw, &c->rscale, &c->partials, &c->dotv, &c->slocs, &c->strc,
+                    &c->rlocs, &c->rmap, &c->smap, &c->traces})
     b->release();
-  for (hipEvent_t e : c->ktime_ev) (void)hipEventDestroy(e);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  c->have_space = c->have_vel = false;
+  c->destroyed = true;
+  ctx_release(c);
   return BRTM_OK;
 }
 
@@ -624,6 +641,15 @@ int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, flo
     s.release();
     s.kind = BRTM_STORE_NONE;
   }
+  // Source points and receivers index the previous grid: a new grid starts
+  // with none (brtm_set_source / brtm_set_receivers give them again).
+  c->nsrc = 0;
+  c->src_plane = false;
+  c->src_z = -1;
+  c->nrec = c->nrx = c->nry = 0;
+  c->rec_plane = false;
+  c->rec_z = -1;
+  c->h_rlocs.clear();
   BRTM_HIP(hipStreamSynchronize(c->stream));
   c->have_space = true;
   c->have_vel = false;
@@ -885,6 +911,10 @@ int brtm_propagate(brtm_ctx *c, int form, int nsteps, float *p_cur, float *p_old
   }
   BRTM_HIP(hipMemcpyAsync(p_cur, cur, fbytes, hipMemcpyDeviceToHost, c->stream));
   BRTM_HIP(hipMemcpyAsync(p_old, old, fbytes, hipMemcpyDeviceToHost, c->stream));
+  // the caller's fields may have had nonzero halos: leave the work buffers
+  // with zero halos, as every propagation expects of them
+  BRTM_HIP(launch_zero_halo(cur, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+  BRTM_HIP(launch_zero_halo(old, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
   BRTM_HIP(hipStreamSynchronize(c->stream));
   return BRTM_OK;
 }

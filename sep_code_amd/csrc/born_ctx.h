@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,13 @@ struct Store {
 }  // namespace brtm
 
 struct brtm_ctx {
+  // Lifetime.  Every vector, operator, solver and network created on the
+  // context holds one reference (brtm::CtxHold) and brtm_create holds one.
+  // brtm_destroy frees the context's device memory at once and drops its
+  // reference; the host record (stream, events) stays until the last holder
+  // is destroyed, so destroying a child after its context is safe.
+  std::atomic<int> refs{1};
+  bool destroyed = false;
   int device = 0;
   int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
@@ -128,6 +136,31 @@ struct brtm_ctx {
 
 namespace brtm {
 
+// Reference counting of a context's host record (see brtm_ctx::refs).
+void ctx_retain(brtm_ctx *c);
+void ctx_release(brtm_ctx *c);
+
+// One reference to a context, held by an object created on it.  Declared as
+// the first member of such an object, so it is released after the object's
+// destructor and its other members (device buffers) are done.
+struct CtxHold {
+  brtm_ctx *c = nullptr;
+  CtxHold() = default;
+  CtxHold(const CtxHold &) = delete;
+  CtxHold &operator=(const CtxHold &) = delete;
+  ~CtxHold() { reset(nullptr); }
+  void reset(brtm_ctx *ctx) {
+    if (ctx) ctx_retain(ctx);
+    if (c) ctx_release(c);
+    c = ctx;
+  }
+};
+
+// BRTM_ESTATE unless the context is live (brtm_destroy not yet called).
+#define BRTM_LIVE(ctx)                                                            \
+  BRTM_REQUIRE((ctx) && !(ctx)->destroyed, BRTM_ESTATE,                          \
+               "the context this object was created on has been destroyed")
+
 // Internal propagation entry points (used by the C ABI and the operator).
 int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int store, int ckpt, float *p0,
                 float *p1);
@@ -153,6 +186,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
 // allocation for an owning vector, or a slice of another vector's storage
 // for a view (a part of a stacked vector), which owns nothing.
 struct brtm_vec {
+  brtm::CtxHold hold;  // owning vectors only; views hold nothing
   brtm_ctx *ctx = nullptr;
   long n = 0;
   float *ptr = nullptr;

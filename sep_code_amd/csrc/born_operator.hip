@@ -14,6 +14,7 @@
 #include "born_ctx.h"
 
 struct brtm_op {
+  brtm::CtxHold hold;
   brtm_ctx *ctx = nullptr;
   brtm_op_desc d{};
   std::vector<int> src_locs, rec_locs;
@@ -32,6 +33,7 @@ namespace brtm {
 
 int check_vec(const brtm_vec *v, long n, const char *what) {
   BRTM_REQUIRE(v && v->p(), BRTM_EINVAL, std::string(what) + " is NULL");
+  BRTM_LIVE(v->ctx);
   BRTM_REQUIRE(v->n == n, BRTM_EINVAL,
                std::string(what) + " has " + std::to_string(v->n) + " elements, expected " +
                    std::to_string(n));
@@ -124,8 +126,10 @@ extern "C" {
 // ---------------------------------------------------------------- vectors
 int brtm_vec_create(brtm_ctx *ctx, long n, brtm_vec **out) {
   BRTM_REQUIRE(ctx && out && n > 0, BRTM_EINVAL, "bad vector arguments");
+  BRTM_LIVE(ctx);
   BRTM_HIP(hipSetDevice(ctx->device));
   brtm_vec *v = new brtm_vec();
+  v->hold.reset(ctx);
   v->ctx = ctx;
   v->n = n;
   int rc = v->buf.alloc(sizeof(float) * (size_t)n);
@@ -146,6 +150,8 @@ int brtm_vec_create(brtm_ctx *ctx, long n, brtm_vec **out) {
 
 int brtm_vec_destroy(brtm_vec *v) {
   if (!v) return BRTM_OK;
+  // the context's host record outlives brtm_destroy while v holds it
+  (void)hipSetDevice(v->ctx->device);
   (void)hipStreamSynchronize(v->ctx->stream);
   delete v;
   return BRTM_OK;
@@ -156,6 +162,7 @@ void *brtm_vec_data(brtm_vec *v) { return v ? v->p() : nullptr; }
 
 int brtm_vec_upload(brtm_vec *v, const float *host) {
   BRTM_REQUIRE(v && host, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(hipMemcpyAsync(v->p(), host, sizeof(float) * (size_t)v->n, hipMemcpyHostToDevice,
                           v->ctx->stream));
   BRTM_HIP(hipStreamSynchronize(v->ctx->stream));
@@ -164,6 +171,7 @@ int brtm_vec_upload(brtm_vec *v, const float *host) {
 
 int brtm_vec_download(const brtm_vec *v, float *host) {
   BRTM_REQUIRE(v && host, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(hipMemcpyAsync(host, v->p(), sizeof(float) * (size_t)v->n, hipMemcpyDeviceToHost,
                           v->ctx->stream));
   BRTM_HIP(hipStreamSynchronize(v->ctx->stream));
@@ -172,12 +180,14 @@ int brtm_vec_download(const brtm_vec *v, float *host) {
 
 int brtm_vec_zero(brtm_vec *v) {
   BRTM_REQUIRE(v, BRTM_EINVAL, "NULL vector");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(hipMemsetAsync(v->p(), 0, sizeof(float) * (size_t)v->n, v->ctx->stream));
   return BRTM_OK;
 }
 
 int brtm_vec_copy(brtm_vec *dst, const brtm_vec *src) {
   BRTM_REQUIRE(dst && src && dst->n == src->n, BRTM_EINVAL, "vector size mismatch");
+  BRTM_LIVE(dst->ctx);
   BRTM_HIP(hipMemcpyAsync(dst->p(), src->p(), sizeof(float) * (size_t)src->n,
                           hipMemcpyDeviceToDevice, dst->ctx->stream));
   return BRTM_OK;
@@ -185,18 +195,21 @@ int brtm_vec_copy(brtm_vec *dst, const brtm_vec *src) {
 
 int brtm_vec_scale(brtm_vec *v, double s) {
   BRTM_REQUIRE(v, BRTM_EINVAL, "NULL vector");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(launch_axpby(v->p(), v->p(), s, 0.0, v->n, v->ctx->stream));
   return BRTM_OK;
 }
 
 int brtm_vec_scale_add(brtm_vec *v, double a, const brtm_vec *x, double b) {
   BRTM_REQUIRE(v && x && v->n == x->n, BRTM_EINVAL, "vector size mismatch");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(launch_axpby(v->p(), x->p(), a, b, v->n, v->ctx->stream));
   return BRTM_OK;
 }
 
 int brtm_vec_add(brtm_vec *v, const brtm_vec *x) {
   BRTM_REQUIRE(v && x && v->n == x->n, BRTM_EINVAL, "vector size mismatch");
+  BRTM_LIVE(v->ctx);
   k_add<<<grid_of(v->n), 256, 0, v->ctx->stream>>>(v->p(), x->p(), v->n);
   BRTM_HIP(hipGetLastError());
   return BRTM_OK;
@@ -204,17 +217,20 @@ int brtm_vec_add(brtm_vec *v, const brtm_vec *x) {
 
 int brtm_vec_mult(brtm_vec *v, const brtm_vec *x) {
   BRTM_REQUIRE(v && x && v->n == x->n, BRTM_EINVAL, "vector size mismatch");
+  BRTM_LIVE(v->ctx);
   BRTM_HIP(launch_mul(v->p(), x->p(), v->n, v->ctx->stream));
   return BRTM_OK;
 }
 
 int brtm_vec_dot(const brtm_vec *a, const brtm_vec *b, double *out) {
   BRTM_REQUIRE(a && b && out && a->n == b->n, BRTM_EINVAL, "vector size mismatch");
+  BRTM_LIVE(a->ctx);
   return brtm::dot(a->ctx, a->p(), b->p(), a->n, out);
 }
 
 int brtm_vec_random(brtm_vec *v, unsigned long long seed) {
   BRTM_REQUIRE(v, BRTM_EINVAL, "NULL vector");
+  BRTM_LIVE(v->ctx);
   k_random<<<grid_of(v->n), 256, 0, v->ctx->stream>>>(v->p(), v->n, seed);
   BRTM_HIP(hipGetLastError());
   return BRTM_OK;
@@ -223,6 +239,7 @@ int brtm_vec_random(brtm_vec *v, unsigned long long seed) {
 // --------------------------------------------------------------- operator
 int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out) {
   BRTM_REQUIRE(ctx && desc && out, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(ctx);
   BRTM_REQUIRE(ctx->have_space && ctx->have_vel, BRTM_ESTATE,
                "the context needs brtm_create_space and brtm_set_velocity first");
   const brtm_op_desc &d = *desc;
@@ -239,6 +256,7 @@ int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out) {
   BRTM_REQUIRE(d.nsrc == 0 || (d.src_locs && d.src_traces), BRTM_EINVAL, "missing sources");
   BRTM_REQUIRE(d.rec_locs, BRTM_EINVAL, "missing receivers");
   brtm_op *op = new brtm_op();
+  op->hold.reset(ctx);
   op->ctx = ctx;
   op->d = d;
   op->nt_fd = (d.nt_data - 1) * d.jt + 1;  // Born/rtm_zero_op_3d.cpp:23
@@ -276,8 +294,11 @@ int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out) {
 
 int brtm_op_destroy(brtm_op *op) {
   if (!op) return BRTM_OK;
+  (void)hipSetDevice(op->ctx->device);
   (void)hipStreamSynchronize(op->ctx->stream);
-  for (int s = 0; s < op->d.nshots; ++s) brtm_release_store(op->ctx, s);
+  // a destroyed context has already freed the shots' stores
+  if (!op->ctx->destroyed)
+    for (int s = 0; s < op->d.nshots; ++s) brtm_release_store(op->ctx, s);
   delete op;
   return BRTM_OK;
 }
@@ -287,6 +308,7 @@ long brtm_op_data_size(const brtm_op *op) { return op ? op->data_size() : 0; }
 
 int brtm_op_prepare(brtm_op *op) {
   BRTM_REQUIRE(op, BRTM_EINVAL, "NULL operator");
+  BRTM_LIVE(op->ctx);
   BRTM_HIP(hipSetDevice(op->ctx->device));
   for (int s = 0; s < op->d.nshots; ++s) {
     OP_TRY(load_shot(op, s));
@@ -299,6 +321,7 @@ int brtm_op_prepare(brtm_op *op) {
 
 int brtm_op_forward(brtm_op *op, int add, const brtm_vec *model, brtm_vec *data) {
   BRTM_REQUIRE(op, BRTM_EINVAL, "NULL operator");
+  BRTM_LIVE(op->ctx);
   BRTM_REQUIRE(op->prepared, BRTM_ESTATE, "brtm_op_prepare has not been called");
   OP_TRY(check_vec(model, op->model_size(), "model"));
   OP_TRY(check_vec(data, op->data_size(), "data"));
@@ -333,6 +356,7 @@ int brtm_op_forward(brtm_op *op, int add, const brtm_vec *model, brtm_vec *data)
 
 int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data) {
   BRTM_REQUIRE(op, BRTM_EINVAL, "NULL operator");
+  BRTM_LIVE(op->ctx);
   BRTM_REQUIRE(op->prepared, BRTM_ESTATE, "brtm_op_prepare has not been called");
   OP_TRY(check_vec(model, op->model_size(), "model"));
   OP_TRY(check_vec(data, op->data_size(), "data"));
@@ -371,6 +395,7 @@ int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data)
 
 int brtm_op_illumination(brtm_op *op, brtm_vec *out) {
   BRTM_REQUIRE(op, BRTM_EINVAL, "NULL operator");
+  BRTM_LIVE(op->ctx);
   BRTM_REQUIRE(op->prepared, BRTM_ESTATE, "brtm_op_prepare has not been called");
   OP_TRY(check_vec(out, op->model_size(), "out"));
   brtm_ctx *c = op->ctx;

@@ -17,6 +17,7 @@
 #include "born_ctx.h"
 
 struct brtm_linop {
+  brtm::CtxHold hold;
   enum Kind { BORN, IDENTITY, DIAG, TMUTE, CALLBACK, CHAIN, COLUMN, BLOCKDIAG };
   Kind kind = IDENTITY;
   brtm_ctx *ctx = nullptr;
@@ -42,6 +43,7 @@ struct brtm_linop {
 };
 
 struct brtm_solver {
+  brtm::CtxHold hold;
   brtm_ctx *ctx = nullptr;
   int kind = BRTM_SOLVER_SIMPLE;
   brtm_linop *oper = nullptr;          // the stacked system
@@ -215,7 +217,9 @@ int apply(brtm_linop *A, bool adjoint, int add, const brtm_vec *in, brtm_vec *ou
 int new_linop(brtm_ctx *ctx, brtm_linop::Kind kind, long nd, long nr, brtm_linop **out) {
   BRTM_REQUIRE(ctx && out, BRTM_EINVAL, "NULL argument");
   BRTM_REQUIRE(nd > 0 && nr > 0, BRTM_EINVAL, "operator sizes must be positive");
+  BRTM_LIVE(ctx);
   brtm_linop *A = new brtm_linop();
+  A->hold.reset(ctx);
   A->kind = kind;
   A->ctx = ctx;
   A->nd = nd;
@@ -586,6 +590,7 @@ int brtm_linop_dot_test(brtm_linop *A, unsigned long long seed, double err[2]) {
 
 int brtm_linop_destroy(brtm_linop *A) {
   if (!A) return BRTM_OK;
+  (void)hipSetDevice(A->ctx->device);
   (void)hipStreamSynchronize(A->ctx->stream);
   delete A;
   return BRTM_OK;
@@ -609,8 +614,10 @@ int brtm_solver_create(int kind, brtm_linop *op, const brtm_vec *data, brtm_lino
   if (kind == BRTM_SOLVER_PREC)
     BRTM_REQUIRE(aux->nr == op->nd, BRTM_EINVAL, "the preconditioner's range must be the model");
   brtm_ctx *c = op->ctx;
+  BRTM_LIVE(c);
   BRTM_HIP(hipSetDevice(c->device));
   brtm_solver *S = new brtm_solver();
+  S->hold.reset(c);
   S->ctx = c;
   S->kind = kind;
   auto fail = [&](int rc) {
@@ -703,6 +710,7 @@ int brtm_solver_create(int kind, brtm_linop *op, const brtm_vec *data, brtm_lino
 int brtm_solver_solve(brtm_solver *S, const brtm_cg_params *prm, double *resid_hist,
                       double *time_hist, brtm_cg_stats *stats) {
   BRTM_REQUIRE(S && prm, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(S->ctx);
   BRTM_REQUIRE(prm->niter >= 0, BRTM_EINVAL, "niter must be >= 0");
   BRTM_REQUIRE(prm->history >= 0 && prm->history <= BRTM_CG_MAX_HISTORY, BRTM_EINVAL,
                "history must be in [0, BRTM_CG_MAX_HISTORY]");
@@ -806,6 +814,7 @@ brtm_vec *brtm_solver_residual(brtm_solver *S) { return S ? S->rr : nullptr; }
 
 int brtm_solver_destroy(brtm_solver *S) {
   if (!S) return BRTM_OK;
+  (void)hipSetDevice(S->ctx->device);
   (void)hipStreamSynchronize(S->ctx->stream);
   delete S;
   return BRTM_OK;

@@ -120,6 +120,7 @@ struct Rng {
 }  // namespace
 
 struct brtm_unet {
+  CtxHold hold;
   brtm_ctx *ctx = nullptr;
   int B = 0, P = 0;
   long P3 = 0;
@@ -287,8 +288,10 @@ int brtm_unet_create(brtm_ctx *ctx, int batch, int patch, unsigned long long see
                      brtm_unet **out) {
   BRTM_REQUIRE(ctx && out && batch >= 1 && patch >= 64 && patch % 64 == 0, BRTM_EINVAL,
                "bad U-Net arguments (patch must be a multiple of 64)");
+  BRTM_LIVE(ctx);
   BRTM_HIP(hipSetDevice(ctx->device));
   brtm_unet *u = new brtm_unet();
+  u->hold.reset(ctx);
   u->ctx = ctx;
   u->B = batch;
   u->odd = batch >= 2;
@@ -378,6 +381,7 @@ int brtm_unet_create(brtm_ctx *ctx, int batch, int patch, unsigned long long see
 
 int brtm_unet_destroy(brtm_unet *u) {
   if (!u) return BRTM_OK;
+  (void)hipSetDevice(u->ctx->device);
   (void)hipStreamSynchronize(u->ctx->stream);
   brtm_vec_destroy(u->split_tmp);
   brtm_vec_destroy(u->znet_prev);
@@ -389,6 +393,7 @@ long brtm_unet_num_params(const brtm_unet *u) { return u ? u->nparams : 0; }
 
 int brtm_unet_get_params(brtm_unet *u, float *host) {
   BRTM_REQUIRE(u && host, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(u->ctx);
   BRTM_HIP(hipMemcpyAsync(host, u->params.ptr, sizeof(float) * u->nparams, hipMemcpyDeviceToHost,
                           u->ctx->stream));
   BRTM_HIP(hipStreamSynchronize(u->ctx->stream));
@@ -397,6 +402,7 @@ int brtm_unet_get_params(brtm_unet *u, float *host) {
 
 int brtm_unet_set_params(brtm_unet *u, const float *host) {
   BRTM_REQUIRE(u && host, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(u->ctx);
   BRTM_HIP(hipStreamSynchronize(u->ctx->stream));
   BRTM_HIP(hipMemcpyAsync(u->params.ptr, host, sizeof(float) * u->nparams, hipMemcpyHostToDevice,
                           u->ctx->stream));
@@ -407,6 +413,7 @@ int brtm_unet_set_params(brtm_unet *u, const float *host) {
 
 int brtm_unet_forward(brtm_unet *u, const float *x, float *y, int nb) {
   BRTM_REQUIRE(u && x && y && nb >= 1 && nb <= u->B, BRTM_EINVAL, "bad forward arguments");
+  BRTM_LIVE(u->ctx);
   U_TRY(forward(u, x, nb));
   BRTM_HIP(hipMemcpyAsync(y, u->out.ptr, sizeof(float) * nb * u->P3, hipMemcpyDeviceToDevice,
                           u->ctx->stream));
@@ -416,6 +423,7 @@ int brtm_unet_forward(brtm_unet *u, const float *x, float *y, int nb) {
 int brtm_unet_loss_grad(brtm_unet *u, const float *x, const float *t, int nb, double *loss,
                         float *grad, int grad_on_host) {
   BRTM_REQUIRE(u && x && t && nb >= 1 && nb <= u->B, BRTM_EINVAL, "bad arguments");
+  BRTM_LIVE(u->ctx);
   U_TRY(loss_grad(u, x, t, nb, loss));
   if (grad) {
     BRTM_HIP(hipMemcpyAsync(grad, u->grads.ptr, sizeof(float) * u->nparams,
@@ -429,6 +437,7 @@ int brtm_unet_loss_grad(brtm_unet *u, const float *x, const float *t, int nb, do
 int brtm_unet_train_step(brtm_unet *u, const float *x, const float *t, int nb, float lr,
                          double *loss) {
   BRTM_REQUIRE(u && x && t && nb >= 1 && nb <= u->B, BRTM_EINVAL, "bad arguments");
+  BRTM_LIVE(u->ctx);
   U_TRY(loss_grad(u, x, t, nb, loss));
   u->step += 1;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -442,6 +451,7 @@ int brtm_unet_train_step(brtm_unet *u, const float *x, const float *t, int nb, f
 int brtm_unet_extract(brtm_unet *u, const float *vol, int m1, int m2, int m3, int o1, int o2,
                       int o3, float scale, float *dst) {
   BRTM_REQUIRE(u && vol && dst, BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(u->ctx);
   k_extract<<<1024, 256, 0, u->ctx->stream>>>(vol, m1, m2, m3, o1, o2, o3, scale, dst, u->P);
   BRTM_HIP(hipGetLastError());
   return BRTM_OK;
@@ -509,12 +519,15 @@ static int apply_signed(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, in
 
 int brtm_unet_apply(brtm_unet *u, const brtm_vec *g, brtm_vec *z, int m1, int m2, int m3,
                     int ov) {
+  BRTM_REQUIRE(u, BRTM_EINVAL, "NULL network");
+  BRTM_LIVE(u->ctx);
   return apply_signed(u, g, z, m1, m2, m3, ov, 1.f);
 }
 
 int brtm_unet_set_grid(brtm_unet *u, int m1, int m2, int m3, int overlap) {
   BRTM_REQUIRE(u && m1 > 0 && m2 > 0 && m3 > 0 && overlap >= 0 && 2 * overlap < u->P,
                BRTM_EINVAL, "bad grid");
+  BRTM_LIVE(u->ctx);
   u->gm1 = m1;
   u->gm2 = m2;
   u->gm3 = m3;
@@ -525,6 +538,7 @@ int brtm_unet_set_grid(brtm_unet *u, int m1, int m2, int m3, int overlap) {
 
 int brtm_unet_set_split_weight(brtm_unet *u, const brtm_vec *w) {
   BRTM_REQUIRE(u, BRTM_EINVAL, "NULL network");
+  BRTM_LIVE(u->ctx);
   if (w && (!u->split_tmp || brtm_vec_size(u->split_tmp) != brtm_vec_size(w))) {
     brtm_vec_destroy(u->split_tmp);
     brtm_vec_destroy(u->znet_prev);
@@ -554,6 +568,7 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
     set_error("brtm_unet_precond: call brtm_unet_set_grid first");
     return BRTM_ESTATE;
   }
+  BRTM_LIVE(u->ctx);
   if (u->split_w) {
     // z = W . (-s net(-W g / s)), s = rms(W g)
     BRTM_REQUIRE(brtm_vec_size(u->split_w) == brtm_vec_size(g), BRTM_EINVAL,

@@ -36,11 +36,11 @@ class BornDevice:
         self._children.add(obj)
 
     def alive(self) -> bool:
-        """False once the context is destroyed.  Children check it before
-        their own destroy call: when the garbage collector finalises a
-        context together with its children (weak references cleared first),
-        the order is arbitrary, and a child left over after the context only
-        drops its handle (its device memory went with the process)."""
+        """False once the context is destroyed.  Children may still be
+        destroyed after that, in any order (the garbage collector finalises
+        a context and its children in arbitrary order): each child holds a
+        reference to the context's host record in the library, and its
+        destroy call frees its own device memory."""
         return getattr(self, "h", None) is not None and bool(self.h.value)
 
     def close(self):

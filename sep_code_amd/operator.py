@@ -162,9 +162,6 @@ class DeviceVector:
             self.h = None
             return
         if getattr(self, "h", None) is not None and self.h.value:
-            if not self.dev.alive():
-                self.h = None
-                return
             call("brtm_vec_destroy", self.h)
             self.h = None
 
@@ -269,9 +266,6 @@ class RtmZeroOp:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            if not self.dev.alive():
-                self.h = None
-                return
             call("brtm_op_destroy", self.h)
             self.h = None
 
@@ -419,9 +413,6 @@ class LinOp:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            if not self.dev.alive():
-                self.h = None
-                return
             call("brtm_linop_destroy", self.h)
             self.h = None
 
@@ -623,9 +614,6 @@ class _Solver:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            if not self.dev.alive():
-                self.h = None
-                return
             call("brtm_solver_destroy", self.h)
             self.h = None
 

@@ -64,9 +64,6 @@ class UNet:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            if not self.dev.alive():
-                self.h = None
-                return
             call("brtm_unet_destroy", self.h)
             self.h = None
 

@@ -449,3 +449,28 @@ def test_adjoint_requires_store(dev):
     dev.source_prop(1, 20, 1, store="none")
     with pytest.raises(BrtmError, match="stored source"):
         dev.rtm_adjoint(1, 20, 1)
+
+
+@pytest.mark.parametrize("ckpt", [2, 7])
+def test_checkpoints_after_propagating_fields_with_halos(dev, ckpt):
+    """brtm_propagate takes whole caller fields, halos included, into the
+    context's work buffers; a checkpointed source propagation afterwards
+    writes those buffers out of place and must not read the stale halos.
+    The image and last fields equal the full FP32 store's bit for bit."""
+    announce(f"checkpoints after nonzero-halo propagate ckpt={ckpt}")
+    P, r = make_shot(dev, nt=60, jimg=1)
+    shape = (P.n[2], P.n[1], P.n[0])
+    d = r.standard_normal((len(P.rlocs), P.ntr_d))
+    d[P.rlocs < 0] = 0
+    out = {}
+    for store in ("f32", "ckpt"):
+        # every cell nonzero, the halo too
+        dev.propagate(0, 2, r.uniform(1, 2, shape), r.uniform(1, 2, shape))
+        p0, p1 = dev.source_prop(1, P.nt, 1, store=store, ckpt=ckpt, get_last=True)
+        dev.set_receivers(P.rlocs, d)
+        dev.zero_image()
+        dev.rtm_adjoint(1, P.nt, 1)
+        out[store] = (dev.get_image(), p0, p1)
+    for a, b in zip(out["ckpt"], out["f32"]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out["f32"][0]).max() > 0
