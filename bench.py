@@ -24,6 +24,16 @@ same stencil step at 512^3, timed on the host cores: the median of three
 samples of ~3.3 s each.  With --workload rtm128 (128^3, nt = 1000) the
 whole shot runs on the CPU instead, three times, and the line also carries
 the relative difference of the CPU image from the GPU image.
+
+lsrtm: the second half of the metric, on the LSRTM workload (lsrtm256:
+256^3, 8 shots, nt = 2500, BF16 source snapshots every 4 steps, data with
+1 % Gaussian noise).  The default run adds --lsrtm-iters iterations each of
+plain CG (cgstep) and of learned flexible CG (shipped network, guard and
+illumination inside the clock): seconds per iteration and both residuals
+after them, |L m - d| / |d| against the noisy data and the misfit to the
+noise-free data.  --workload lsrtm256 runs plain CG, the illumination
+diagonal and learned flexible CG to the 1e-2 tolerance instead and reports
+the time to it (value = learned seconds, lower is better).
 """
 from __future__ import annotations
 
@@ -131,7 +141,7 @@ def cpu_model() -> str:
 
 def _omp_threads():
     """Host threads for the CPU restatement; set before libgomp loads."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
@@ -229,6 +239,120 @@ def cpu_full_shot(w, D, img_gpu, runs=3):
             "image_rel_diff_vs_gpu": float(f"{diff:.3e}")}
 
 
+LSRTM_WORKLOAD = ("lsrtm256: LSRTM, 256x256x256 FP32, 8 shots, nt=2500, BF16 source "
+                  "snapshots every 4 steps, Born data + 1 % Gaussian noise")
+
+
+def _lsrtm_setup(log):
+    from sep_code_amd import configs, lsrtm
+    from sep_code_amd.device import BornDevice
+    t0 = time.perf_counter()
+    w = configs.lsrtm()
+    dev = BornDevice(0)
+    op, tprep = lsrtm.setup_operator(dev, w)
+    d, floor, e = lsrtm.observed_data(op, w, keep_noise=True)
+    log(f"lsrtm256 set up in {time.perf_counter() - t0:.1f}s (source fields {tprep:.1f}s), "
+        f"noise |e|/|d| = {floor:.4f}")
+    return w, dev, op, d, floor, e
+
+
+def lsrtm_slice(iters, log):
+    """`iters` iterations each of plain CG and learned flexible CG on the
+    LSRTM workload from m = 0: seconds per iteration (the solver loop's own
+    clock, preconditioner inference and guard included; the illumination
+    weight, computed once, reported beside it) and both residuals after
+    `iters` iterations, recomputed with the plain FP32 operator and FP64
+    norms."""
+    from sep_code_amd import lsrtm
+    from sep_code_amd.lsrtm_bench import DEFAULT_WEIGHTS
+    from sep_code_amd.unet import UNet
+    w, dev, op, d, floor, e = _lsrtm_setup(log)
+
+    def rec(r, extra=None):
+        out = {"iterations": r.iterations, "s_per_iter": round(r.seconds / r.iterations, 4),
+               "data_rel_resid": round(r.data_rel_resid_recomputed, 6),
+               "misfit_noise_free": round(r.rel_resid_recomputed, 6)}
+        out.update(extra or {})
+        return out
+
+    res = {"workload": LSRTM_WORKLOAD, "noise_rel": round(floor, 6),
+           "residuals": "data_rel_resid = |L m - d| / |d| (noisy data as given); "
+                        "misfit_noise_free = |L m - (d - e)| / |d - e|",
+           "time_to_tolerance": "python bench.py --workload lsrtm256"}
+    r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=iters, rel_tol=0.0, name="cg", noise=e)
+    m.close()
+    res["cg"] = rec(r)
+    log(f"lsrtm plain CG: {iters} it, {r.seconds / iters:.3f} s/it, |Lm-d|/|d| "
+        f"{r.data_rel_resid_recomputed:.4f}, noise-free misfit {r.rel_resid_recomputed:.4f}")
+    net = UNet(dev, batch=8, patch=64, seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    net.set_odd(True)
+    net.set_guard(0.8, 0.05)
+    t0 = time.perf_counter()
+    W = lsrtm.illumination_weight(op, 0.01, power=0.5)
+    tw = time.perf_counter() - t0
+    pre = net.precond(lsrtm.model_box(w)[:3], overlap=8, split_weight=W)
+    r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=iters, rel_tol=0.0, name="fpcg", noise=e,
+                           precond=pre, history=4)
+    m.close()
+    res["fpcg"] = rec(r, {"illumination_s": round(tw, 3), "history": 4,
+                          "guard_diagonal_steps": net.guard_count()})
+    log(f"lsrtm learned FCG: {iters} it, {r.seconds / iters:.3f} s/it, |Lm-d|/|d| "
+        f"{r.data_rel_resid_recomputed:.4f}, noise-free misfit {r.rel_resid_recomputed:.4f}")
+    net.close()
+    W.close()
+    dev.close()
+    return res
+
+
+def lsrtm_main(a):
+    """--workload lsrtm256: plain CG, the illumination diagonal and learned
+    flexible CG to the tolerance; the line's value is the learned run's
+    time to it.  The roofline object covers the BF16 imaging step, the
+    adjoint's dominant kernel (26 algorithmic bytes per cell)."""
+    from sep_code_amd.lsrtm_bench import compare
+    w, dev, op, d, floor, e = _lsrtm_setup(log)
+    t0 = time.perf_counter()
+    out = compare(dev, w, op, d, e, floor, niter=a.lsrtm_niter, tol=1e-2, diag_eps="0.01",
+                  quiet=False, log=log)
+    el = time.perf_counter() - t0
+    # the imaging step alone, 200 launches between HIP events on the library
+    # stream (the solves launch it millions of times)
+    kcount = 200
+    k_ms = dev.time_step(0, 1, 2, kcount)
+    achieved = 26.0 * w.ncell / (k_ms * 1e-3) / 1e9
+    fp = out.get("fpcg", {})
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(w, a.cpu_seconds)
+        cpu.pop("extrapolated_shot", None)
+        cpu["sample"] += ("; one CG iteration advances 4 fields per shot (source field "
+                          "stored: Born forward, then the adjoint), 8 shots")
+        it_s = 4 * len(w.shots) * w.ncell * w.nt / (cpu["value"] * 1e9)
+        cpu["extrapolated_cg_iteration_s"] = {
+            "seconds": round(it_s, 1),
+            "basis": "extrapolated, not measured: the sampled per-step rate over 4 x 8 x "
+                     f"{w.nt} steps, imaging and the Born source not counted"}
+    line = {
+        "metric": "RTM Gpts/s (% HBM roofline); LSRTM time-to-1e-2 residual, learned vs plain CG",
+        "value": round(fp.get("seconds", float("nan")), 2), "unit": "s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(el * 1e3, 1), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": LSRTM_WORKLOAD, "grid": list(w.n), "nt": w.nt,
+                   "shots": len(w.shots), "store": w.store, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "brtm::k_step_x4r VL step with fused imaging from BF16 "
+                               "snapshots (26 B per cell)",
+                     "kernel_ms": round(k_ms, 4), "launches": kcount},
+        "cpu_baseline": cpu, "lsrtm": out,
+    }
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +372,11 @@ def main():
                          "rtm128; at 512^3 it takes hours)")
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
                     help="HBM bytes per stencil launch measured with rocprofv3 --pmc")
+    ap.add_argument("--lsrtm-iters", type=int, default=5,
+                    help="iterations of plain CG and of learned flexible CG on lsrtm256 "
+                         "added to the RTM line (0: none)")
+    ap.add_argument("--lsrtm-niter", type=int, default=400,
+                    help="iteration limit of each solver with --workload lsrtm256")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -255,6 +384,10 @@ def main():
                                    "sits in one MI355X's HBM; multi-GPU sharding is by shot "
                                    "and is not implemented here", "n_gpus": a.gpus}))
         sys.exit(2)
+
+    if a.workload == "lsrtm256":
+        lsrtm_main(a)
+        return
 
     from sep_code_amd import configs
     t0 = time.perf_counter()
@@ -327,8 +460,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "image_finite": finite,
     }
-    print(json.dumps(out), flush=True)
     dev.close()
+    if a.lsrtm_iters > 0 and w.name == "rtm512":
+        out["lsrtm"] = lsrtm_slice(a.lsrtm_iters, log)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -128,6 +128,11 @@ typedef struct {
    * that changes between iterations (a network) it keeps the search
    * directions L-conjugate to the last h steps. */
   int history;
+  /* Optional (NULL: none), niter+1 entries: |rr| / |d| before the first and
+   * after every iteration, the relative residual of the data as given.
+   * Without an offset it equals the judged residual; with one it is the
+   * residual of the noisy data, |L m - d| / |d|. */
+  double *data_resid_hist;
 } brtm_cg_params;
 
 #define BRTM_CG_MAX_HISTORY 8
@@ -139,7 +144,8 @@ typedef struct {
   int converged;            /* rel residual reached rel_tol                     */
   int breakdown;            /* cgstep breakdown (gg.gg or ss.ss < 1e-20)        */
   double seconds;           /* wall-clock of the iteration loop                 */
-  double rel_resid;         /* final running |rr|/|d|                           */
+  double rel_resid;         /* final judged residual (|rr|/|d| without offset)   */
+  double data_rel_resid;    /* final |rr|/|d|                                   */
 } brtm_cg_stats;
 
 /* Solve min |L m - d| from m = 0 (Born/lin_solver.cpp:79-101, rr = -d).

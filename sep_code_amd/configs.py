@@ -14,8 +14,8 @@ Workloads (compute grids include the sponge and the 4-cell zero halo):
   512 x 512 receivers, dt = 1 ms, nt = 4000, FP32 checkpoints every 50 steps;
 * ``lsrtm256`` -- 256^3, d = 12.5 m, dipping layers (2-10 degrees) plus
   sparse reflectivity, 8 shots on a 2 x 4 surface grid, Ricker 15 Hz,
-  nt = 2500, Born data plus 1 % Gaussian noise (sigma = 1 % of the peak
-  absolute sample of the clean data).
+  nt = 2500, Born data plus 1 % Gaussian noise (sigma = 1 % of the RMS of
+  the clean data, so |e| / |d_clean| = 1 %; lsrtm.observed_data).
 
 The geometry follows the reference's conventions (model origin at cell
 nbound = sponge + 4 of the compute grid, Born/source_func_3d.cpp:91-98;

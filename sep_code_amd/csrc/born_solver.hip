@@ -750,10 +750,14 @@ int brtm_solver_solve(brtm_solver *S, const brtm_cg_params *prm, double *resid_h
     *out = S->jnorm > 0 ? std::sqrt(std::max(rr_rr + 2 * re + ee, 0.0)) / S->jnorm : 0.0;
     return r2;
   };
+  // |rr| / |d|: the residual of the data as given (noisy when e is set)
+  auto data_rel = [&](double rr_rr) { return dnorm > 0 ? std::sqrt(rr_rr) / dnorm : 0.0; };
   double rr_rr = 0;
   LS_TRY(brtm_vec_dot(S->rr, S->rr, &rr_rr));
   LS_TRY(judged(rr_rr, &st.rel_resid));
+  st.data_rel_resid = data_rel(rr_rr);
   if (resid_hist) resid_hist[0] = st.rel_resid;
+  if (prm->data_resid_hist) prm->data_resid_hist[0] = st.data_rel_resid;
   if (time_hist) time_hist[0] = 0.0;
   int rc = BRTM_OK;
   for (int iter = 0; iter < prm->niter; ++iter) {
@@ -773,7 +777,9 @@ int brtm_solver_solve(brtm_solver *S, const brtm_cg_params *prm, double *resid_h
     rc = brtm_vec_dot(S->rr, S->rr, &rr_rr);
     if (rc == BRTM_OK) rc = judged(rr_rr, &st.rel_resid);
     if (rc != BRTM_OK) break;
+    st.data_rel_resid = data_rel(rr_rr);
     if (resid_hist) resid_hist[iter + 1] = st.rel_resid;
+    if (prm->data_resid_hist) prm->data_resid_hist[iter + 1] = st.data_rel_resid;
     if (time_hist) time_hist[iter + 1] = elapsed();
     if (prm->verbose)
       std::fprintf(stderr, "cg iter %d alpha=%g beta=%g rel_resid=%.6e t=%.2fs\n", iter, alfa,

@@ -25,12 +25,88 @@ import os
 import sys
 import time
 
-import numpy as np
-
-from . import configs, lsrtm
+from . import _lib, configs, lsrtm
 from .device import BornDevice
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "precond_split_e0.01.npy")
+
+
+def _entry(r, extra=None):
+    """The JSON record of one solver run: iterations and seconds to the
+    tolerance on the judged (noise-free) misfit, and both residual
+    definitions at the stop, running and recomputed with the plain FP32
+    operator and FP64 norms."""
+    out = {"iterations": r.iterations, "converged": r.converged, "seconds": r.seconds,
+           "judged": r.rel_resid_recomputed, "data_rel_resid": r.data_rel_resid_recomputed,
+           "hist": r.resid_hist[::5], "data_hist": r.data_resid_hist[::5]}
+    tol_hit = [i for i, x in enumerate(r.data_resid_hist) if x <= 1e-2]
+    out["data_resid_to_1e-2"] = (
+        {"iterations": tol_hit[0], "seconds": r.time_hist[tol_hit[0]]} if tol_hit else None)
+    out.update(extra or {})
+    return out
+
+
+def compare(dev, w, op, d, e, floor, *, niter=400, tol=1e-2, weights=DEFAULT_WEIGHTS, batch=8,
+            even=False, guard="0.8,0.05", history=4, diag_eps="", split_eps=0.01,
+            skip_plain=False, quiet=True, log=None):
+    """Plain CG, optionally illumination-diagonal PCG for each eps in the
+    comma list diag_eps, and learned flexible CG with the network in
+    `weights` ('' skips it), each from m = 0 on the prepared operator op
+    with data d = L m_true + e.  Returns the comparison as a dict."""
+    log = log or (lambda m: None)
+    store = {v: k for k, v in _lib.STORE_NAMES.items()}[op.desc.store]
+    out = {"workload": w.name, "grid": list(w.n), "nt": w.nt, "shots": len(w.shots),
+           "store": store, "jimg": w.jimg, "tol": tol, "noise_rel": floor,
+           "stop_rule": "misfit to the noise-free data |L m - (d - e)| / |d - e| <= tol; "
+                        "data_rel_resid is |L m - d| / |d| against the noisy data"}
+    if not skip_plain:
+        r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=niter, rel_tol=tol, name="cg", noise=e,
+                               verbose=not quiet)
+        m.close()
+        out["cg"] = _entry(r)
+        log(f"plain CG: {r.iterations} it, {r.seconds:.1f}s, judged "
+            f"{r.rel_resid_recomputed:.4e}, |Lm-d|/|d| {r.data_rel_resid_recomputed:.4e}")
+    for eps in [float(v) for v in diag_eps.split(",") if v]:
+        t0 = time.perf_counter()
+        wv = lsrtm.illumination_weight(op, eps)
+        tw = time.perf_counter() - t0
+        r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=niter, rel_tol=tol, name="diag", noise=e,
+                               precond=lsrtm.diagonal_precond(dev, wv), verbose=not quiet)
+        m.close()
+        out[f"diag_{eps:g}"] = _entry(r, {"seconds": r.seconds + tw})
+        log(f"illumination-diagonal PCG eps={eps:g}: {r.iterations} it, {r.seconds + tw:.1f}s, "
+            f"judged {r.rel_resid_recomputed:.4e}")
+        wv.close()
+    if weights:
+        from .unet import UNet
+        net = UNet(dev, batch=batch, patch=64, seed=1)
+        net.load(weights)
+        net.set_odd(not even)
+        rep, mn = (float(v) for v in guard.split(","))
+        net.set_guard(rep, mn)
+        dims = lsrtm.model_box(w)[:3]
+        t0 = time.perf_counter()
+        W = lsrtm.illumination_weight(op, split_eps, power=0.5) if split_eps > 0 else None
+        tw = time.perf_counter() - t0
+        pre = net.precond(dims, overlap=8, split_weight=W)
+        r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=niter, rel_tol=tol, name="fpcg", noise=e,
+                               precond=pre, history=history, verbose=not quiet)
+        m.close()
+        out["fpcg"] = _entry(r, {"history": history, "split_eps": split_eps,
+                                 "seconds": r.seconds + tw, "illumination_seconds": tw,
+                                 "fallbacks": r.fallbacks,
+                                 "guard_diagonal_steps": net.guard_count(), "odd": not even,
+                                 "guard": guard})
+        log(f"learned FPCG: {r.iterations} it, {r.seconds + tw:.1f}s, "
+            f"judged {r.rel_resid_recomputed:.4e}, fallbacks {r.fallbacks}, "
+            f"guard {net.guard_count()}")
+        net.close()
+        if W is not None:
+            W.close()
+        if "cg" in out and out["cg"]["converged"] and r.converged:
+            out["speedup"] = out["cg"]["seconds"] / out["fpcg"]["seconds"]
+    return out
 
 
 def main():
@@ -40,7 +116,7 @@ def main():
     ap.add_argument("--seed", type=int, default=77)
     ap.add_argument("--niter", type=int, default=400)
     ap.add_argument("--tol", type=float, default=1e-2)
-    ap.add_argument("--weights", default=os.path.join(HERE, "weights", "precond_split_e0.01.npy"),
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
                     help="trained network ('' to skip the learned run)")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--skip-plain", action="store_true")
@@ -58,6 +134,8 @@ def main():
     ap.add_argument("--split-eps", type=float, default=0.01,
                     help="> 0: the learned run is z = W P(W g), W = (I + eps max I)^-1/2 "
                          "(the network trained with the same split)")
+    ap.add_argument("--store", default="",
+                    help="source-wavefield store of the operator (default: the workload's)")
     ap.add_argument("--op-timing", type=int, default=0,
                     help="> 0: only time that many Born forward / adjoint applications of "
                          "the operator (wall clock per shot-step) and exit")
@@ -67,13 +145,13 @@ def main():
     t0 = time.perf_counter()
     w = configs.lsrtm(seed=a.seed, n_side=a.n, nt=a.nt)
     dev = BornDevice(0)
-    op, tprep = lsrtm.setup_operator(dev, w)
+    op, tprep = lsrtm.setup_operator(dev, w, store=a.store or None)
     if a.op_timing > 0:
         m, d = op.model_vector().random(1), op.data_vector()
         steps = len(w.shots) * a.nt
         res = {"workload": w.name, "grid": list(w.n), "nt": a.nt, "shots": len(w.shots),
-               "store": w.store, "jimg": w.jimg, "prepare_s": tprep, "forward_us": [],
-               "adjoint_us": []}
+               "store": a.store or w.store, "jimg": w.jimg, "prepare_s": tprep,
+               "forward_us": [], "adjoint_us": []}
         for _ in range(a.op_timing):
             dev.sync()
             t0 = time.perf_counter()
@@ -91,52 +169,10 @@ def main():
     d, floor, e = lsrtm.observed_data(op, w, keep_noise=True)
     log(f"{w.name}: {len(w.shots)} shots, nt={a.nt}, prepared in {tprep:.1f}s, "
         f"noise |e|/|d| = {floor:.4f}, setup {time.perf_counter() - t0:.1f}s")
-    out = {"workload": w.name, "grid": list(w.n), "nt": a.nt, "shots": len(w.shots),
-           "store": w.store, "jimg": w.jimg, "tol": a.tol, "noise_rel": floor}
-    if not a.skip_plain:
-        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="cg",
-                               noise=e, verbose=not a.quiet)
-        out["cg"] = {"iterations": r.iterations, "converged": r.converged,
-                     "seconds": r.seconds, "judged": r.rel_resid_recomputed,
-                     "hist": r.resid_hist[::5]}
-        log(f"plain CG: {r.iterations} it, {r.seconds:.1f}s, judged {r.rel_resid_recomputed:.4e}")
-    for eps in [float(v) for v in a.diag_eps.split(",") if v]:
-        t0 = time.perf_counter()
-        wv = lsrtm.illumination_weight(op, eps)
-        tw = time.perf_counter() - t0
-        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="diag",
-                               noise=e, precond=lsrtm.diagonal_precond(dev, wv),
-                               verbose=not a.quiet)
-        out[f"diag_{eps:g}"] = {"iterations": r.iterations, "converged": r.converged,
-                                "seconds": r.seconds + tw, "judged": r.rel_resid_recomputed,
-                                "hist": r.resid_hist[::5]}
-        log(f"illumination-diagonal PCG eps={eps:g}: {r.iterations} it, {r.seconds + tw:.1f}s, "
-            f"judged {r.rel_resid_recomputed:.4e}")
-        wv.close()
-    if a.weights:
-        from .unet import UNet
-        net = UNet(dev, batch=a.batch, patch=64, seed=1)
-        net.load(a.weights)
-        net.set_odd(not a.even)
-        rep, mn = (float(v) for v in a.guard.split(","))
-        net.set_guard(rep, mn)
-        dims = lsrtm.model_box(w)[:3]
-        t0 = time.perf_counter()
-        W = lsrtm.illumination_weight(op, a.split_eps, power=0.5) if a.split_eps > 0 else None
-        tw = time.perf_counter() - t0
-        pre = net.precond(dims, overlap=8, split_weight=W)
-        r, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=a.niter, rel_tol=a.tol, name="fpcg",
-                               noise=e, precond=pre, history=a.history, verbose=not a.quiet)
-        out["fpcg"] = {"history": a.history, "split_eps": a.split_eps,
-                       "iterations": r.iterations, "converged": r.converged,
-                       "seconds": r.seconds + tw, "illumination_seconds": tw,
-                       "judged": r.rel_resid_recomputed,
-                       "fallbacks": r.fallbacks, "guard_diagonal_steps": net.guard_count(),
-                       "odd": not a.even, "guard": a.guard, "hist": r.resid_hist[::5]}
-        log(f"learned FPCG: {r.iterations} it, {r.seconds + tw:.1f}s, "
-            f"judged {r.rel_resid_recomputed:.4e}, fallbacks {r.fallbacks}")
-        if "cg" in out and out["cg"]["converged"] and r.converged:
-            out["speedup"] = out["cg"]["seconds"] / out["fpcg"]["seconds"]
+    out = compare(dev, w, op, d, e, floor, niter=a.niter, tol=a.tol, weights=a.weights,
+                  batch=a.batch, even=a.even, guard=a.guard, history=a.history,
+                  diag_eps=a.diag_eps, split_eps=a.split_eps, skip_plain=a.skip_plain,
+                  quiet=a.quiet, log=log)
     print(json.dumps(out), flush=True)
     dev.close()
 

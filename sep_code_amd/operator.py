@@ -47,12 +47,12 @@ APPLY_FN = C.CFUNCTYPE(_I, _P, _I, _P, _P)
 
 class CgParams(C.Structure):
     _fields_ = [("niter", _I), ("rel_tol", _D), ("precond", PRECOND_FN), ("precond_user", _P),
-                ("verbose", _I), ("offset", _P), ("history", _I)]
+                ("verbose", _I), ("offset", _P), ("history", _I), ("data_resid_hist", _P)]
 
 
 class CgStats(C.Structure):
     _fields_ = [("iterations", _I), ("fallbacks", _I), ("converged", _I), ("breakdown", _I),
-                ("seconds", _D), ("rel_resid", _D)]
+                ("seconds", _D), ("rel_resid", _D), ("data_rel_resid", _D)]
 
 
 _SIGS = {
@@ -342,20 +342,26 @@ class SolveResult:
     rel_resid: float
     resid_hist: np.ndarray
     time_hist: np.ndarray
+    # |L m - d| / |d| of the data as given (the judged residual without an
+    # offset; the noisy-data residual with one)
+    data_rel_resid: float = float("nan")
+    data_resid_hist: np.ndarray | None = None
 
 
 def cg_solve(op: RtmZeroOp, data: DeviceVector, model: DeviceVector, niter: int,
              rel_tol=0.0, verbose=False):
     """brtm_cg_solve: plain cgstep on the Born operator from m = 0 in one
     call (the C entry point for callers without the solver objects)."""
-    p = CgParams(int(niter), float(rel_tol), PRECOND_FN(), None, int(verbose), None, 0)
+    dh = np.zeros(niter + 1)
+    p = CgParams(int(niter), float(rel_tol), PRECOND_FN(), None, int(verbose), None, 0,
+                 ptr(dh))
     rh = np.zeros(niter + 1)
     th = np.zeros(niter + 1)
     st = CgStats()
     call("brtm_cg_solve", op.h, data.h, model.h, C.byref(p), ptr(rh), ptr(th), C.byref(st))
     k = st.iterations + 1
     return SolveResult(st.iterations, st.fallbacks, bool(st.converged), bool(st.breakdown),
-                       st.seconds, st.rel_resid, rh[:k], th[:k])
+                       st.seconds, st.rel_resid, rh[:k], th[:k], st.data_rel_resid, dh[:k])
 
 
 # --- operators (my_operator and its kinds) -----------------------------------
@@ -594,14 +600,16 @@ class _Solver:
         p = CgParams(int(niter), float(rel_tol),
                      self._precond_cb if self._precond_cb is not None else PRECOND_FN(),
                      self._precond_user, int(self.verbose),
-                     offset.h if offset is not None else None, self.history)
+                     offset.h if offset is not None else None, self.history, None)
         rh = np.zeros(niter + 1)
         th = np.zeros(niter + 1)
+        dh = np.zeros(niter + 1)
+        p.data_resid_hist = ptr(dh)
         st = CgStats()
         call("brtm_solver_solve", self.h, C.byref(p), ptr(rh), ptr(th), C.byref(st))
         k = st.iterations + 1
         return SolveResult(st.iterations, st.fallbacks, bool(st.converged), bool(st.breakdown),
-                           st.seconds, st.rel_resid, rh[:k], th[:k])
+                           st.seconds, st.rel_resid, rh[:k], th[:k], st.data_rel_resid, dh[:k])
 
     def return_model(self) -> DeviceVector:
         v = DeviceVector(self.dev, self.op.domain_size)
