@@ -288,6 +288,11 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     S.count = (nt - 1) / jimg + 1;
     BRTM_TRY(S.data.alloc(fmt_bytes(fmt) * (size_t)c->ncell * S.count));
     if (fmt != FMT_F32) BRTM_TRY(S.scale.alloc(sizeof(float) * (size_t)c->n3 * S.count));
+    // FP32 snapshots are written by the steps themselves and read back as
+    // the next step's input, so their halos must be zero
+    if (fmt == FMT_F32)
+      BRTM_HIP(launch_zero_halo(S.data.as<float>(), c->ncell, S.count, c->n1, c->n2, c->n3,
+                                c->stream));
   } else if (kind == BRTM_STORE_CKPT) {
     S.count = ck_count;
     BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
@@ -335,6 +340,11 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
         out = ck_cur((it + 2) / ckpt) + c->ncell;      // s^{it+1} = s^{K ckpt - 1}
       else if (old != wx && old != wy)
         out = cur == wx ? wy : wx;                     // s^{it-1} is a checkpoint: keep it
+    } else if (fmt == FMT_F32) {
+      if (it % jimg == 0)
+        out = S.data.as<float>() + (size_t)c->ncell * (it / jimg);  // snapshot k itself
+      else if (old != wx && old != wy)
+        out = cur == wx ? wy : wx;                     // s^{it-1} is a snapshot: keep it
     }
     bool fused = false;
     BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, out, nullptr, nullptr, nullptr,
@@ -342,13 +352,9 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     if (!fused) BRTM_TRY(inject_source(c, it, jt, out));
     if (fmt != FMT_NONE && it % jimg == 0) {
       const int k = it / jimg;
-      if (fmt == FMT_F32) {
-        BRTM_HIP(hipMemcpyAsync(S.data.as<char>() + fbytes * (size_t)k, out, fbytes,
-                                hipMemcpyDeviceToDevice, c->stream));
-      } else {
+      if (fmt != FMT_F32)  // (FP32: the step wrote snapshot k)
         BRTM_HIP(launch_encode(fmt, out, S.data.as<char>() + 2 * (size_t)c->ncell * k,
                                S.scale.as<float>() + (size_t)c->n3 * k, n12, c->n3, c->stream));
-      }
     }
     old = cur;
     cur = out;

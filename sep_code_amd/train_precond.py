@@ -17,6 +17,8 @@ and Adam on random 64^3 patches, with random sign (the map is odd) and,
 optionally (--scale-aug), a random pair amplitude.
 
     python -m sep_code_amd.train_precond --pairs 96 --out weights.npy
+    python -m sep_code_amd.train_precond --pairs 400 --workers 3 --gen-seconds 900 \
+        --init weights.npy --seed0 2000 --out weights2.npy      # continued, more models
 """
 from __future__ import annotations
 
@@ -88,8 +90,59 @@ def normalise(a):
     return (a / r).astype(np.float32) if r > 0 else a
 
 
-def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print, scale_aug=1.0):
+def generate(seeds, workers=1, log=print, deadline=None, **kw):
+    """make_pairs for every seed, on `workers` device contexts of GPU 0
+    driven from as many host threads (a 128^3 step fills only part of the
+    GPU, so independent models overlap); stops taking new seeds once the
+    perf_counter `deadline` has passed.  Returns [(x, y, dims)] in seed
+    order and the seeds done."""
+    import queue
+    import threading
+    todo = queue.Queue()
+    for i, sd in enumerate(seeds):
+        todo.put((i, sd))
+    done, lock, errors = {}, threading.Lock(), []
+    t0 = time.perf_counter()
+
+    def work():
+        dev = BornDevice(0)
+        try:
+            while not errors:
+                if deadline is not None and time.perf_counter() > deadline:
+                    return
+                try:
+                    i, sd = todo.get_nowait()
+                except queue.Empty:
+                    return
+                ps, dims, rr = make_pairs(dev, sd, **kw)
+                with lock:
+                    done[i] = (sd, [(x, y, dims) for x, y in ps])
+                    log(f"model {len(done)}/{len(seeds)} (seed {sd}): {len(ps)} pairs, "
+                        f"residual after 20 CG {rr:.3f} ({time.perf_counter() - t0:.1f}s)")
+        except Exception as e:  # reported after the join
+            errors.append(e)
+        finally:
+            dev.close()
+
+    threads = [threading.Thread(target=work) for _ in range(max(1, workers))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    pairs, used = [], []
+    for i in sorted(done):
+        used.append(done[i][0])
+        pairs.extend(done[i][1])
+    return pairs, used
+
+
+def train(dev, pairs, steps=2000, batch=4, lr=1e-3, seed=0, log=print, scale_aug=1.0,
+          init=None):
     net = UNet(dev, batch=batch, patch=64, seed=seed + 1)
+    if init:
+        net.load(init)
     rng = np.random.default_rng(seed)
     vols = []
     for x, y, dims in pairs:
@@ -135,28 +188,32 @@ def main():
                     help="> 1: random pair amplitude, log-uniform in [1/s, s]")
     ap.add_argument("--split-eps", type=float, default=0.0,
                     help="> 0: pairs for the illumination-split preconditioner")
+    ap.add_argument("--workers", type=int, default=1,
+                    help="device contexts generating models concurrently")
+    ap.add_argument("--gen-seconds", type=float, default=0.0,
+                    help="> 0: take no new model after this many seconds of generation")
+    ap.add_argument("--init", default="",
+                    help="start from these network parameters (continued training)")
     a = ap.parse_args()
     states = tuple(int(v) for v in a.states.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    dev = BornDevice(0)
-    pairs = []
     t0 = time.perf_counter()
-    for i in range(a.pairs):
-        ps, dims, rr = make_pairs(dev, a.seed0 + i, n_side=a.n, nt=a.nt, states=states,
-                                  split_eps=a.split_eps)
-        pairs.extend((x, y, dims) for x, y in ps)
-        log(f"model {i + 1}/{a.pairs}: {len(ps)} pairs, residual after 20 CG {rr:.3f} "
-            f"({time.perf_counter() - t0:.1f}s)")
+    deadline = t0 + a.gen_seconds if a.gen_seconds > 0 else None
+    pairs, used = generate(range(a.seed0, a.seed0 + a.pairs), workers=a.workers, log=log,
+                           deadline=deadline, n_side=a.n, nt=a.nt, states=states,
+                           split_eps=a.split_eps)
     if a.save_pairs:
         np.savez(a.save_pairs, **{f"x{i}": p[0] for i, p in enumerate(pairs)},
                  **{f"y{i}": p[1] for i, p in enumerate(pairs)})
     tgen = time.perf_counter() - t0
+    dev = BornDevice(0)
     net, hist = train(dev, pairs, steps=a.steps, batch=a.batch, lr=a.lr, log=log,
-                      scale_aug=a.scale_aug)
+                      scale_aug=a.scale_aug, init=a.init or None)
     net.save(a.out)
-    print(json.dumps({"models": a.pairs, "pairs": len(pairs), "states": list(states),
-                      "split_eps": a.split_eps, "scale_aug": a.scale_aug,
-                      "gen_seconds": tgen, "steps": a.steps,
+    print(json.dumps({"models": len(used), "seeds": [min(used), max(used)] if used else [],
+                      "pairs": len(pairs), "states": list(states), "split_eps": a.split_eps,
+                      "scale_aug": a.scale_aug, "workers": a.workers, "init": a.init,
+                      "gen_seconds": tgen, "steps": a.steps, "lr": a.lr,
                       "loss_first100": float(np.mean(hist[:100])),
                       "loss_last100": float(np.mean(hist[-100:]))}), flush=True)
     net.close()

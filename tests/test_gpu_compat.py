@@ -118,7 +118,8 @@ def test_reference_abi_source_prop_closed_form(dev):
     row = np.zeros(8, np.float32)
     row[3] = 1
     lib.transfer_sinc_table_s(8, 1, (C.c_void_p * 1)(row.ctypes.data))
-    lib.transfer_source_func(9, ntb, slocs.ctypes.data, strc.astype(np.float32).ctypes.data)
+    strc = np.ascontiguousarray(strc, np.float32)  # kept alive across the call
+    lib.transfer_source_func(9, ntb, slocs.ctypes.data, strc.ctypes.data)
     p0 = np.zeros((n3, n2, n1), np.float32)
     p1 = np.zeros_like(p0)
     lib.source_prop(n1, n2, n3, False, True, p0.ctypes.data, p1.ctypes.data, 1, 9, nt)
