@@ -669,3 +669,85 @@ def test_source_geometry_hand_example():
 
 def test_receiver_index_hand_example():
     assert O.receiver_index(10, 12, [2.7], [3.1], [5.9])[0] == 2 + 3 * 10 + (5 + 4) * 120
+
+
+# --- FP32 over long runs: the error does not accumulate, and the summed ----
+# --- (Henrici) form of the update does not reduce it ------------------------
+def _fp32_runs(n=36, nt=1000, jimg=100, bc_a=8.0, bc_b=0.02):
+    """One point source in a heterogeneous 36^3 box with a sponge, advanced
+    nt steps three ways: the FP64 oracle, and in numpy FP32 both the plain
+    VL update p' = w(V L p + 2p) - w^2 p_old (the GPU's, 16 B per cell) and
+    its summed form (Henrici), which carries d = p - p_old:
+    d' = w V L p - (1-w)^2 p + w^2 d, p' = p + d' (20 B per cell).
+    Returns the two FP32 errors at every jimg-th step relative to the FP64
+    field's largest norm, and that field's norm at the same steps."""
+    d = (10.0, 10.0, 10.0)
+    r = rng(9)
+    V = ((2000.0 * 0.001) ** 2 * (1 + 0.2 * r.random((n, n, n)))).astype(np.float32)
+    t = np.arange(nt + 8) * 0.001
+    a = (np.pi * 25.0 * (t - 0.06)) ** 2
+    tr = np.zeros((1, nt + 8))
+    tr[0, 4:] = ((1 - 2 * a) * np.exp(-a))[:nt + 4]
+    c3 = n // 2
+    loc = np.array([c3 + c3 * n + c3 * n * n], np.int32)
+    P = O.Problem((n, n, n), d, V.astype(np.float64), bc_a, bc_b, src_locs=loc, src_tr=tr,
+                  nt=nt, jt=1, jimg=jimg)
+    s64, _ = P.source_prop()
+    c = O.coeffs(*d).astype(np.float32)
+    wt = O.sponge_table(bc_a, bc_b)
+    i = np.arange(n)
+    e1 = np.minimum(i - 4, n - 4 - i)
+    E = np.minimum(np.minimum(e1[:, None, None], e1[None, :, None]), e1[None, None, :])
+    Wf = np.where((E >= 0) & (E < len(wt)), wt[np.clip(E, 0, len(wt) - 1)], 1.0)
+    U = (slice(4, -4),) * 3
+    Vu, Wu = V[U], Wf.astype(np.float32)[U]
+
+    def lap(p):
+        acc = c[0] * p[U]
+        for k in range(1, 5):
+            acc = acc + c[1 + 3 * (k - 1)] * (p[4:-4, 4:-4, 4 + k:n - 4 + k] + p[4:-4, 4:-4, 4 - k:n - 4 - k])
+            acc = acc + c[2 + 3 * (k - 1)] * (p[4:-4, 4 + k:n - 4 + k, 4:-4] + p[4:-4, 4 - k:n - 4 - k, 4:-4])
+            acc = acc + c[3 + 3 * (k - 1)] * (p[4 + k:n - 4 + k, 4:-4, 4:-4] + p[4 - k:n - 4 - k, 4:-4, 4:-4])
+        return acc
+
+    z = np.zeros((n, n, n), np.float32)
+    p, po, q, dl = z.copy(), z.copy(), z.copy(), z.copy()
+    ip = np.unravel_index(loc[0], (n, n, n))
+    e_plain, e_sum = [], []
+    one, two = np.float32(1), np.float32(2)
+    for it in range(nt):
+        f = np.float32(tr[0, it + 3])
+        pn = z.copy()
+        pn[U] = Wu * (Vu * lap(p) + two * p[U]) - (Wu * Wu) * po[U]
+        pn[ip] += f
+        po, p = p, pn
+        dn = z.copy()
+        dn[U] = Wu * Vu * lap(q) - (one - Wu) ** 2 * q[U] + (Wu * Wu) * dl[U]
+        dn[ip] += f
+        q, dl = q + dn, dn
+        if it % jimg == 0:
+            ref = s64[it // jimg].reshape(n, n, n)
+            e_plain.append(np.linalg.norm(p - ref))
+            e_sum.append(np.linalg.norm(q - ref))
+    norms = np.array([np.linalg.norm(s) for s in s64])
+    peak = norms.max()
+    return np.array(e_plain) / peak, np.array(e_sum) / peak, norms / peak
+
+
+def test_fp32_error_does_not_accumulate_and_summing_does_not_help():
+    """The FP32 error of the leapfrog, measured against the FP64 oracle and
+    relative to the field's largest norm, peaks while the wavefield is
+    energetic (a few 1e-6) and then falls as the field leaves through the
+    sponge: it does not grow with the number of steps.  Its ratio to the
+    decaying current norm does grow, which is what a per-step relative error
+    shows.  The summed form, whose point is to stop rounding from being
+    integrated twice, leaves the error where it was (within 30 %), so the
+    GPU keeps the plain 16 B/cell update."""
+    ep, es, norms = _fp32_runs()
+    assert ep.max() < 1e-5 and es.max() < 1e-5
+    k = int(np.argmax(ep))
+    assert norms[k] > 1e-2                      # the peak error is in the energetic phase
+    assert ep[-1] < 0.1 * ep.max() and es[-1] < 0.1 * es.max()
+    assert norms[-1] < 1e-3                      # the field itself has decayed by 1000x
+    assert ep[-1] / norms[-1] > 10 * ep[1] / norms[1]   # relative to itself, it grows
+    assert 0.7 * ep.max() < es.max() < 1.3 * ep.max()
