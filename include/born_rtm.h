@@ -215,6 +215,20 @@ int brtm_set_stream_loads(brtm_ctx *ctx, int mode);
  * scale the counters of the other kernels to HBM bytes. */
 int brtm_copy_bench(brtm_ctx *ctx, double bytes, int nrep, double *avg_ms);
 
+/* FP64 inner product sum_i (double)a[i] * (double)b[i] of two device FP32
+ * arrays of n elements (n >= 0), into the host double *out, on the
+ * context's stream; the same kernels and the same bits as brtm_vec_dot
+ * (born_op.h).  No counterpart in the reference: this is brtm_vec_dot for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc, hipFree or allocating routine: its scratch, the 1024 partial
+ * sums and the result (8,200 bytes), is a static device array of the
+ * library, one per GPU, held until the process ends.  Its use does not grow
+ * with n, so every n >= 0 fits; a negative n is refused.  Calls that
+ * overlap on one GPU (several host threads, or contexts of one device on
+ * different streams) share that array and race: serialise them. */
+int brtm_dot_noalloc(brtm_ctx *ctx, const float *a, const float *b, long n, double *out);
+
 /* Device synchronisation of the library stream. */
 int brtm_sync(brtm_ctx *ctx);
 

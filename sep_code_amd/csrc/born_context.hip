@@ -9,10 +9,19 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/born_rtm.h"
 #include "born_ctx.h"
+
+// Scratch of the allocation-free FP64 dot product (brtm_dot_noalloc): the
+// kDotBlocks per-block partial sums, then the result.  One copy per GPU,
+// held until the process ends.  In host code `scratch` is only a stand-in
+// for the symbol: its address on a device comes from hipGetSymbolAddress
+// with that device current (kept in brtm_ctx::noalloc_scratch).
+constexpr int kNoallocDoubles = brtm::kDotBlocks + 1;  // 8,200 bytes
+static __device__ __align__(256) double scratch[kNoallocDoubles];
 
 namespace brtm {
 
@@ -533,6 +542,29 @@ int dot(brtm_ctx *c, const float *a, const float *b, long n, double *out) {
   return BRTM_OK;
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};  // address of `scratch` on each device
+
+// The address of `scratch` on `device`, which must be the current device:
+// looked up once per device and kept.
+int noalloc_scratch(int device, double **out) {
+  BRTM_REQUIRE(device >= 0 && device < kMaxDevices, BRTM_EINVAL,
+               "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    BRTM_REQUIRE(e == hipSuccess && p, BRTM_EHIP,
+                 std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch[device];
+  return BRTM_OK;
+}
+}  // namespace
+
 void ctx_retain(brtm_ctx *c) { c->refs.fetch_add(1); }
 
 void ctx_release(brtm_ctx *c) {
@@ -576,6 +608,12 @@ int brtm_create(int device, brtm_ctx **out) {
     delete c;
     set_error("stream/event creation failed");
     return BRTM_EHIP;
+  }
+  // the allocation-free entry point's scratch on this device (current above)
+  const int rc = noalloc_scratch(device, &c->noalloc_scratch);
+  if (rc != BRTM_OK) {
+    ctx_release(c);
+    return rc;
   }
   *out = c;
   return BRTM_OK;
@@ -1049,6 +1087,23 @@ int brtm_kernel_timing_result(brtm_ctx *c, double *total_ms, int *count) {
   }
   *total_ms = t;
   *count = (int)(c->ktime_used / 2);
+  return BRTM_OK;
+}
+
+int brtm_dot_noalloc(brtm_ctx *c, const float *a, const float *b, long n, double *out) {
+  BRTM_REQUIRE(c && out && (n == 0 || (a && b)), BRTM_EINVAL, "NULL argument");
+  BRTM_LIVE(c);
+  BRTM_REQUIRE(n >= 0, BRTM_EINVAL, "n must be >= 0");
+  BRTM_REQUIRE(c->noalloc_scratch, BRTM_ESTATE, "the context has no scratch address");
+  static_assert(kDotBlocks + 1 <= kNoallocDoubles, "the dot's partial sums exceed the scratch");
+  BRTM_HIP(hipSetDevice(c->device));
+  // the same two kernels as brtm_vec_dot, on the static scratch: every
+  // partial sum and the result are written before they are read, so
+  // nothing needs clearing between calls
+  double *partials = c->noalloc_scratch, *result = c->noalloc_scratch + kDotBlocks;
+  BRTM_HIP(launch_dot(a, b, n, partials, result, c->stream));
+  BRTM_HIP(hipMemcpyAsync(out, result, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  BRTM_HIP(hipStreamSynchronize(c->stream));
   return BRTM_OK;
 }
 

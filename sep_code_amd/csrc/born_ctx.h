@@ -99,6 +99,9 @@ struct brtm_ctx {
   bool born_direct_lv = false;
   int seg_slots = 0;
   brtm::DevBuf partials, dotv;
+  // this device's copy of the allocation-free entry point's scratch array
+  // (its device address, from hipGetSymbolAddress at brtm_create)
+  double *noalloc_scratch = nullptr;
 
   // interpolation tables [jt][8]
   std::vector<float> sinc[2];
