@@ -172,9 +172,11 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     }
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
   }
-  // receivers inside the step: four-cell ring kernel, one receiver plane
+  // receivers inside the step: the four-cell ring kernel, or the one-cell
+  // kernel's 8-row small-grid shape; one receiver plane
   const bool src = rec == REC_SOURCE;
-  const bool in_step = rec != REC_NONE && fused && c->rec_fusion && xl == 4 && ring &&
+  const bool rec_kernel = (xl == 4 && ring) || (xl == 1 && !ring && ty == 8);
+  const bool in_step = rec != REC_NONE && fused && c->rec_fusion && rec_kernel &&
                        form == FORM_VL &&
                        (src ? c->src_plane && c->nsrc > 0 : c->rec_plane && c->nrec > 0);
   if (in_step) {
@@ -196,7 +198,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   else if (xl == 2)
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
-    BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream));
+    BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream, kmode));
   if (timed) {
     BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
     c->ktime_used += 2;
@@ -284,8 +286,11 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
   // allocated afresh.
   const bool ck_direct = kind == BRTM_STORE_CKPT && ckpt >= 2;
   const int ck_count = kind == BRTM_STORE_CKPT ? (nt + ckpt - 1) / ckpt : 0;
-  const bool keep = ck_direct && S.kind == BRTM_STORE_CKPT && S.ckpt_halo_zero &&
-                    S.data.bytes >= 2 * fbytes * (size_t)ck_count;
+  // FP32 snapshots are written by the steps too: the same reuse applies
+  const bool direct = ck_direct || fmt == FMT_F32;
+  const size_t need = ck_direct ? 2 * fbytes * (size_t)ck_count
+                                : fbytes * (size_t)((nt - 1) / jimg + 1);
+  const bool keep = direct && S.kind == kind && S.halo_zero && S.data.bytes >= need;
   if (!keep) S.release();
   S.kind = kind;
   S.nt = nt;
@@ -298,10 +303,12 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     BRTM_TRY(S.data.alloc(fmt_bytes(fmt) * (size_t)c->ncell * S.count));
     if (fmt != FMT_F32) BRTM_TRY(S.scale.alloc(sizeof(float) * (size_t)c->n3 * S.count));
     // FP32 snapshots are written by the steps themselves and read back as
-    // the next step's input, so their halos must be zero
-    if (fmt == FMT_F32)
-      BRTM_HIP(launch_zero_halo(S.data.as<float>(), c->ncell, S.count, c->n1, c->n2, c->n3,
-                                c->stream));
+    // the next step's input, so their halos must be zero (once per buffer)
+    if (fmt == FMT_F32 && !keep) {
+      BRTM_HIP(launch_zero_halo(S.data.as<float>(), c->ncell, (int)(S.data.bytes / fbytes),
+                                c->n1, c->n2, c->n3, c->stream));
+      S.halo_zero = true;
+    }
   } else if (kind == BRTM_STORE_CKPT) {
     S.count = ck_count;
     BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
@@ -326,7 +333,7 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
       const size_t nf = S.data.bytes / fbytes;  // whole buffer, also beyond count pairs
       BRTM_HIP(launch_zero_halo(ck_cur(0) + 2 * c->ncell, c->ncell, (int)nf - 2, c->n1, c->n2,
                                 c->n3, c->stream));
-      S.ckpt_halo_zero = true;
+      S.halo_zero = true;
     }
     cur = ck_cur(0);
     old = ck_cur(0) + c->ncell;

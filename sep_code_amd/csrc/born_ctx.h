@@ -55,13 +55,14 @@ struct Store {
   int nt = 0, jt = 1, jimg = 1, ckpt = 0, count = 0;
   DevBuf data;     // snapshots [count][ncell] or checkpoints [count][2][ncell]
   DevBuf scale;    // [count][n3] slice scales (16-bit formats)
-  // checkpoints: every pair's halo is zero (the steps writing into the
-  // pairs never touch it), so a later propagation can reuse the buffer
-  bool ckpt_halo_zero = false;
+  // checkpoints and FP32 snapshots: every field slot's halo is zero (the
+  // steps writing into the slots never touch it), so a later propagation of
+  // the same kind can reuse the buffer as it is
+  bool halo_zero = false;
   void release() {
     data.release();
     scale.release();
-    ckpt_halo_zero = false;
+    halo_zero = false;
   }
 };
 

@@ -14,7 +14,8 @@ enum StepForm { FORM_VL = 0, FORM_LV = 1 };
 //   ACT_IMG  img += cimg . s . p_new        (zero-lag imaging, VL only)
 //   ACT_BORN p_new += cimg . s . m          (Born secondary source)
 enum StepAct { ACT_NONE = 0, ACT_IMG = 1, ACT_BORN = 2 };
-// Receivers handled inside the step (the four-cell ring kernel):
+// Receivers handled inside the step (the four-cell ring kernel, and the
+// one-cell kernel's 8-row small-grid shape):
 //   REC_INJECT   pn += sum_k rsinc[k] rtr(r, rid+k) before imaging and the store
 //   REC_EXTRACT  rtr(r, rid+k) += rsinc[k] . pn . (rscale[r] or 1)
 // for the cells of plane rz whose rmap entry is a receiver index r >= 0.
@@ -58,7 +59,10 @@ struct StepParams {
   float rsinc[8];
 };
 
-hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
+// rec (RecMode): the receivers or source points of one plane inside the
+// step; VL form with 8-row tiles (the small-grid shape) only.
+hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st,
+                       int rec = REC_NONE);
 // The VL step with a six-plane LDS ring (halos loaded with the z-queue).
 hipError_t launch_step_ring(int act, int fmt, StepParams P, int ty_tile, hipStream_t st);
 // The VL step with two consecutive x per lane (128-wide tiles, n1 even).

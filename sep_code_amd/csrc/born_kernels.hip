@@ -64,10 +64,13 @@ __device__ __forceinline__ int edist(int i, int n) { return min(i - 4, n - 4 - i
 // ---------------------------------------------------------------------------
 // The fused stencil step.
 // ---------------------------------------------------------------------------
-template <int FORM, int TY, int RPT, int FMT, int ACT>
+template <int FORM, int TY, int RPT, int FMT, int ACT, int REC = REC_NONE>
 __global__ void __launch_bounds__(64 * (TY / RPT))
 k_step(const StepParams P) {
   static_assert((ACT == ACT_NONE) == (FMT == FMT_NONE), "an action needs a snapshot format");
+  static_assert(REC == REC_NONE || FORM == FORM_VL, "receivers inside the VL step only");
+  static_assert(!(REC == REC_INJECT && ACT == ACT_BORN) && !(REC == REC_EXTRACT && ACT == ACT_IMG),
+                "injection goes with the adjoint, extraction with Born modelling");
   constexpr int NWV = TY / RPT;  // waves per workgroup
   constexpr int HY = 8 / NWV;    // y-halo rows loaded per wave
   static_assert(8 % NWV == 0, "waves must divide the 8 halo rows");
@@ -232,6 +235,28 @@ k_step(const StepParams P) {
           pn = acc + 2.f * w * rq[r][0] - (w * w) * cpo[r];
         }
         if constexpr (ACT == ACT_BORN) pn = fmaf(P.cimg * cS[r], cX[r], pn);
+        if constexpr (REC != REC_NONE) {
+          // the receiver plane: k_inject / k_extract's arithmetic for the
+          // (at most one) receiver of this cell, before imaging and the store
+          if (z == P.rz) {
+            const int ri = P.rmap[y * n1 + x];
+            if (ri >= 0) {
+              if constexpr (REC == REC_INJECT) {
+                float a = 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                  if (P.rsinc[k] != 0.f)
+                    a = fmaf(P.rsinc[k], P.rtr[ri * P.rsp + (P.rid + k) * P.rst], a);
+                pn += a;
+              } else {
+                const float v = P.rscale ? pn * P.rscale[ri] : pn;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                  if (P.rsinc[k] != 0.f) P.rtr[ri * P.rsp + (P.rid + k) * P.rst] += v * P.rsinc[k];
+              }
+            }
+          }
+        }
         // The new field and the image are not read again in this launch:
         // stream them past the caches.
         __builtin_nontemporal_store(pn, P.pn + i);
@@ -1648,14 +1673,14 @@ hipError_t launch_x4_t(int act, int fmt, const StepParams &P, hipStream_t st, in
   return hipErrorInvalidValue;
 }
 
-template <int FORM, int TY, int RPT, int ACT>
+template <int FORM, int TY, int RPT, int ACT, int REC = REC_NONE>
 hipError_t launch_step_a(int fmt, const StepParams &P, hipStream_t st) {
   const dim3 block(64, TY / RPT);
   const dim3 grid(P.ntx * P.ntz * P.nty);
   switch (fmt) {
-    case FMT_F32: k_step<FORM, TY, RPT, FMT_F32, ACT><<<grid, block, 0, st>>>(P); break;
-    case FMT_BF16: k_step<FORM, TY, RPT, FMT_BF16, ACT><<<grid, block, 0, st>>>(P); break;
-    case FMT_F16: k_step<FORM, TY, RPT, FMT_F16, ACT><<<grid, block, 0, st>>>(P); break;
+    case FMT_F32: k_step<FORM, TY, RPT, FMT_F32, ACT, REC><<<grid, block, 0, st>>>(P); break;
+    case FMT_BF16: k_step<FORM, TY, RPT, FMT_BF16, ACT, REC><<<grid, block, 0, st>>>(P); break;
+    case FMT_F16: k_step<FORM, TY, RPT, FMT_F16, ACT, REC><<<grid, block, 0, st>>>(P); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1672,6 +1697,24 @@ hipError_t launch_step_t(int act, int fmt, const StepParams &P, hipStream_t st) 
   if (act == ACT_BORN) return launch_step_a<FORM, TY, RPT, ACT_BORN>(fmt, P, st);
   if constexpr (FORM == FORM_VL) {
     if (act == ACT_IMG) return launch_step_a<FORM, TY, RPT, ACT_IMG>(fmt, P, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// The small-grid shape (VL, 8-row tiles) with the receivers or the source
+// points of one plane inside the step.
+template <int REC>
+hipError_t launch_step_rec_t(int act, int fmt, const StepParams &P, hipStream_t st) {
+  if (act == ACT_NONE) {
+    if (fmt != FMT_NONE) return hipErrorInvalidValue;
+    k_step<FORM_VL, 8, 2, FMT_NONE, ACT_NONE, REC><<<dim3(P.ntx * P.nty * P.ntz), dim3(64, 4), 0,
+                                                    st>>>(P);
+    return hipGetLastError();
+  }
+  if constexpr (REC == REC_EXTRACT) {
+    if (act == ACT_BORN) return launch_step_a<FORM_VL, 8, 2, ACT_BORN, REC>(fmt, P, st);
+  } else {
+    if (act == ACT_IMG) return launch_step_a<FORM_VL, 8, 2, ACT_IMG, REC>(fmt, P, st);
   }
   return hipErrorInvalidValue;
 }
@@ -1719,8 +1762,17 @@ void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw) {
   P.ntz = (nz + P.zc - 1) / P.zc;
 }
 
-hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st) {
+hipError_t launch_step(int form, int act, int fmt, StepParams P, int ty_tile, hipStream_t st,
+                       int rec) {
   if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 9) return hipErrorInvalidValue;
+  if (rec != REC_NONE) {
+    if (form != FORM_VL || ty_tile != 8 || !P.rmap || !P.rtr || P.nrec <= 0 || P.rz < 4 ||
+        P.rz >= P.n3 - 4)
+      return hipErrorInvalidValue;
+    if (rec == REC_INJECT) return launch_step_rec_t<REC_INJECT>(act, fmt, P, st);
+    if (rec == REC_EXTRACT) return launch_step_rec_t<REC_EXTRACT>(act, fmt, P, st);
+    return hipErrorInvalidValue;
+  }
   if (form == FORM_VL) {
     if (ty_tile == 32) return launch_step_t<FORM_VL, 32, 4>(act, fmt, P, st);
     if (ty_tile == 8) return launch_step_t<FORM_VL, 8, 2>(act, fmt, P, st);

@@ -267,18 +267,21 @@ def test_actions_variants(dev, store, v):
         assert rel(out[v][1], P.rtm_adjoint(Do, snaps)) < 1e-4
 
 
+@pytest.mark.parametrize("variant", [(4, 1), (0, -1)], ids=["x4ring", "small_grid"])
 @pytest.mark.parametrize("jt,store", [(1, "f32"), (2, "f32"), (1, "bf16")])
-def test_receiver_fusion(dev, jt, store):
-    """Receivers injected / extracted inside the four-cell ring step: traces
-    bit-identical to the separate extraction kernel, images to FP32 rounding
-    of the separate injection kernel, both against the oracle."""
-    announce(f"receiver fusion jt={jt} store={store}")
+def test_receiver_fusion(dev, jt, store, variant):
+    """Receivers (and the source points) injected / extracted inside the step
+    -- the four-cell ring kernel, or at this size with the automatic shape the
+    one-cell kernel's 8-row small-grid tiles: traces bit-identical to the
+    separate extraction kernel, images to FP32 rounding of the separate
+    injection kernel, both against the oracle."""
+    announce(f"receiver fusion jt={jt} store={store} variant={variant}")
     P, r = make_shot(dev, n=(52, 44, 40), jt=jt, jimg=2)
     m = random_field(r, P.n)
     snaps, _ = P.source_prop()
     Do = P.born_forward(m.ravel(), snaps)
     out = {}
-    set_variant(dev, (4, 1))
+    set_variant(dev, variant)
     try:
         for fuse in (0, 1):
             dev.set_receiver_fusion(fuse)
@@ -301,12 +304,13 @@ def test_receiver_fusion(dev, jt, store):
         assert rel(out[1][1], P.rtm_adjoint(Do, snaps)) < 1e-4
 
 
+@pytest.mark.parametrize("variant", [(4, 1), (0, -1)], ids=["x4ring", "small_grid"])
 @pytest.mark.parametrize("layout", ["duplicate", "two_planes"])
-def test_receivers_off_one_plane_fall_back(dev, layout):
+def test_receivers_off_one_plane_fall_back(dev, layout, variant):
     """Receivers sharing a cell, or on two depths, cannot go through the
     per-plane receiver map: the separate kernels run, and results still match
     the oracle."""
-    announce(f"receiver fallback {layout}")
+    announce(f"receiver fallback {layout} variant={variant}")
     P, r = make_shot(dev, n=(52, 44, 40), jimg=2)
     n1, n2 = P.n[0], P.n[1]
     rl = P.rlocs.copy()
@@ -320,7 +324,7 @@ def test_receivers_off_one_plane_fall_back(dev, layout):
     m = random_field(r, P.n)
     snaps, _ = Q.source_prop()
     Do = Q.born_forward(m.ravel(), snaps)
-    set_variant(dev, (4, 1))
+    set_variant(dev, variant)
     try:
         dev.set_receivers(rl, None, ntr=P.ntr_d, nx=len(rl), ny=1)
         dev.source_prop(1, P.nt, 2, store="f32")
