@@ -310,11 +310,15 @@ def lsrtm_main(a):
     flexible CG to the tolerance; the line's value is the learned run's
     time to it.  The roofline object covers the BF16 imaging step, the
     adjoint's dominant kernel (26 algorithmic bytes per cell)."""
-    from sep_code_amd.lsrtm_bench import compare
+    from sep_code_amd.lsrtm_bench import DEFAULT_WEIGHTS, compare
+    a.lsrtm_weights = a.lsrtm_weights or DEFAULT_WEIGHTS
     w, dev, op, d, floor, e = _lsrtm_setup(log)
     t0 = time.perf_counter()
-    out = compare(dev, w, op, d, e, floor, niter=a.lsrtm_niter, tol=1e-2, diag_eps="0.01",
-                  quiet=False, log=log)
+    solvers = set(a.lsrtm_solvers.split(","))
+    out = compare(dev, w, op, d, e, floor, niter=a.lsrtm_niter, tol=1e-2,
+                  diag_eps="0.01" if "diag" in solvers else "",
+                  weights=a.lsrtm_weights if "fpcg" in solvers else "",
+                  skip_plain="cg" not in solvers, quiet=False, log=log)
     el = time.perf_counter() - t0
     # the imaging step alone, 200 launches between HIP events on the library
     # stream (the solves launch it millions of times)
@@ -377,6 +381,11 @@ def main():
                          "added to the RTM line (0: none)")
     ap.add_argument("--lsrtm-niter", type=int, default=400,
                     help="iteration limit of each solver with --workload lsrtm256")
+    ap.add_argument("--lsrtm-solvers", default="cg,diag,fpcg",
+                    help="with --workload lsrtm256: which of plain CG (cg), the illumination "
+                         "diagonal (diag) and learned flexible CG (fpcg) to run")
+    ap.add_argument("--lsrtm-weights", default="",
+                    help="network parameters of the learned run (default: the shipped ones)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -431,6 +440,11 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": ("--traffic-bytes of this run" if a.traffic_bytes else
+                               "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch of "
+                               "this kernel at this shape, from a separate counter run of the "
+                               "same build (profiles/stencil512_traffic.md), not measured in "
+                               "this run" if traffic else "not measured for this shape"),
             "kernel": {1: "brtm::k_step (VL, fused sponge)",
                        2: "brtm::k_step_x2 (VL, 2 cells/lane, fused sponge)",
                        4: "brtm::k_step_x4r (VL, 4 cells/lane, six-plane halo ring, fused sponge)"

@@ -3,13 +3,14 @@
 Training pairs are made by this library on seeded random models (layered,
 dipping and Gaussian-anomaly velocities with sparse reflectivity, the
 acquisition of the LSRTM workload: 8 shots on a 2 x 4 surface grid, 1 %
-noise).  One 20-iteration cgstep LSRTM run per model gives
+noise).  One cgstep LSRTM run of N iterations per model (--cg-iters, 20 by
+default) gives
   input  = RTM image       L^T d                (= -g_0)
-  target = LSRTM image     m_20
+  target = LSRTM image     m_N
 and, from the same run, pairs for the later gradients the preconditioner
 meets inside the solver:
   input  = -g_k = L^T (d - L m_k)   (RTM image of the data residual)
-  target = m_20 - m_k               (the rest of the 20-iteration update)
+  target = m_N - m_k                (the rest of the N-iteration update)
 for k in --states (k = 0 is the plain pair).  Each volume is normalised by
 its own RMS (the CG search fixes the step length, so only the direction of
 P(g) matters).  The network is trained with its own forward/backward kernels
@@ -43,7 +44,7 @@ def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(
     split_eps > 0: the run is CG preconditioned by the illumination diagonal
     W^2, W = (I + eps max I)^-1/2, and the pairs are amplitude-balanced for
     the split preconditioner z = W P(W g):
-      input  = W (-g_k),   target = (m_20 - m_k) / W."""
+      input  = W (-g_k),   target = (m_N - m_k) / W, N = cg_iters."""
     w = configs.lsrtm(seed=seed, n_side=n_side, nt=nt, anomalies=int(seed % 4))
     op, _ = lsrtm.setup_operator(dev, w)
     d, _ = lsrtm.observed_data(op, w, noise_frac=noise, seed=seed + 99)
@@ -70,8 +71,8 @@ def make_pairs(dev, seed, n_side=128, nt=1300, cg_iters=20, noise=0.01, states=(
 
     solver = SimpleSolver(cgstep(), d, op, precond=record)
     res = solver.solve(cg_iters)
-    m20 = solver.model.numpy()
-    pairs = [(-kept[k][0], m20 - kept[k][1]) for k in states if k in kept]
+    mN = solver.model.numpy()
+    pairs = [(-kept[k][0], mN - kept[k][1]) for k in states if k in kept]
     if W is not None:
         wh = W.numpy()
         pairs = [(wh * x, y / wh) for x, y in pairs]
@@ -118,7 +119,8 @@ def generate(seeds, workers=1, log=print, deadline=None, **kw):
                 with lock:
                     done[i] = (sd, [(x, y, dims) for x, y in ps])
                     log(f"model {len(done)}/{len(seeds)} (seed {sd}): {len(ps)} pairs, "
-                        f"residual after 20 CG {rr:.3f} ({time.perf_counter() - t0:.1f}s)")
+                        f"residual after {kw.get('cg_iters', 20)} CG {rr:.3f} "
+                        f"({time.perf_counter() - t0:.1f}s)")
         except Exception as e:  # reported after the join
             errors.append(e)
         finally:
@@ -182,6 +184,8 @@ def main():
     ap.add_argument("--save-pairs", default="")
     ap.add_argument("--states", default="0,4,9,14",
                     help="CG iterations whose gradient / remaining update form pairs")
+    ap.add_argument("--cg-iters", type=int, default=20,
+                    help="length of each model's CG run (the targets are m_cg_iters - m_k)")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--scale-aug", type=float, default=1.0,
@@ -201,7 +205,7 @@ def main():
     deadline = t0 + a.gen_seconds if a.gen_seconds > 0 else None
     pairs, used = generate(range(a.seed0, a.seed0 + a.pairs), workers=a.workers, log=log,
                            deadline=deadline, n_side=a.n, nt=a.nt, states=states,
-                           split_eps=a.split_eps)
+                           split_eps=a.split_eps, cg_iters=a.cg_iters)
     if a.save_pairs:
         np.savez(a.save_pairs, **{f"x{i}": p[0] for i, p in enumerate(pairs)},
                  **{f"y{i}": p[1] for i, p in enumerate(pairs)})
@@ -213,6 +217,7 @@ def main():
     print(json.dumps({"models": len(used), "seeds": [min(used), max(used)] if used else [],
                       "pairs": len(pairs), "states": list(states), "split_eps": a.split_eps,
                       "scale_aug": a.scale_aug, "workers": a.workers, "init": a.init,
+                      "cg_iters": a.cg_iters,
                       "gen_seconds": tgen, "steps": a.steps, "lr": a.lr,
                       "loss_first100": float(np.mean(hist[:100])),
                       "loss_last100": float(np.mean(hist[-100:]))}), flush=True)
