@@ -26,7 +26,7 @@ whole shot runs on the CPU instead, three times, and the line also carries
 the relative difference of the CPU image from the GPU image.
 
 lsrtm: the second half of the metric, on the LSRTM workload (lsrtm256:
-256^3, 8 shots, nt = 2500, BF16 source snapshots every 4 steps, data with
+256^3, 8 shots, nt = 2500, FP16 source snapshots every 4 steps, data with
 1 % Gaussian noise).  The default run adds --lsrtm-iters iterations each of
 plain CG (cgstep) and of learned flexible CG (shipped network, guard and
 illumination inside the clock): seconds per iteration and both residuals
@@ -239,7 +239,7 @@ def cpu_full_shot(w, D, img_gpu, runs=3):
             "image_rel_diff_vs_gpu": float(f"{diff:.3e}")}
 
 
-LSRTM_WORKLOAD = ("lsrtm256: LSRTM, 256x256x256 FP32, 8 shots, nt=2500, BF16 source "
+LSRTM_WORKLOAD = ("lsrtm256: LSRTM, 256x256x256 FP32, 8 shots, nt=2500, FP16 source "
                   "snapshots every 4 steps, Born data + 1 % Gaussian noise")
 
 
@@ -323,7 +323,7 @@ def lsrtm_main(a):
     # the imaging step alone, 200 launches between HIP events on the library
     # stream (the solves launch it millions of times)
     kcount = 200
-    k_ms = dev.time_step(0, 1, 2, kcount)
+    k_ms = dev.time_step(0, 1, 3, kcount)
     achieved = 26.0 * w.ncell / (k_ms * 1e-3) / 1e9
     fp = out.get("fpcg", {})
     cpu = None
@@ -348,7 +348,7 @@ def lsrtm_main(a):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "brtm::k_step_x4r VL step with fused imaging from BF16 "
+                     "kernel": "brtm::k_step_x4r VL step with fused imaging from FP16 "
                                "snapshots (26 B per cell)",
                      "kernel_ms": round(k_ms, 4), "launches": kcount},
         "cpu_baseline": cpu, "lsrtm": out,

@@ -186,7 +186,7 @@ def rtm512(seed: int = 4096, nt: int = 4000, n_side: int = 512, rec_step: int = 
 
 
 def lsrtm(seed: int = 77, n_side: int = 256, nt: int = 2500, nshots=(2, 4), rec_step=2,
-          d=12.5, jimg=4, store="bf16", anomalies: int = 0) -> Workload:
+          d=12.5, jimg=4, store="f16", anomalies: int = 0) -> Workload:
     n = (n_side, n_side, n_side)
     dd, dt, sponge = (d, d, d), 0.001, max(8, n_side // 12)
     nbound = sponge + 4
