@@ -209,6 +209,14 @@ int brtm_set_prefetch(brtm_ctx *ctx, int slices);
  * size.  No counterpart in the reference; the same results. */
 int brtm_set_stream_loads(brtm_ctx *ctx, int mode);
 
+/* Launch each propagation loop (source propagation, Born modelling, the
+ * adjoint) as one graph of its captured kernel launches instead of launch by
+ * launch: 1 on, 0 off, -1 (the default) on for grids of at most 4 M cells,
+ * where the time between dependent launches is a sizeable part of a step
+ * (128^3).  Per-launch kernel timing (brtm_kernel_timing) launches eagerly.
+ * No counterpart in the reference; the same results. */
+int brtm_set_graphs(brtm_ctx *ctx, int mode);
+
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms
  * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE

@@ -94,6 +94,10 @@ struct brtm_ctx {
   bool rec_fusion = true;  // receivers inside the four-cell ring step where they allow it
   int prefetch = 0;        // load lookahead of the plain four-cell ring step: 0 auto, 1, 2
   int stream_loads = -1;   // streaming loads in the four-cell ring step: -1 auto, 0, 1
+  int graphs = -1;         // propagation loops launched as graphs: -1 auto, 0, 1
+  // executable graphs launched and not yet known to be finished: freed once
+  // the stream has been synchronised
+  std::vector<hipGraphExec_t> graph_execs;
 
   brtm::DevBuf d_wtab, V, fa, fb, fc, fd, img, mdl, seg;
   brtm::DevBuf mvw, rscale;  // V.w.m and 1/(V.w) at receivers (Born path on u = V.w.q)

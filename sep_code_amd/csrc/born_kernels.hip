@@ -1813,8 +1813,9 @@ hipError_t launch_receiver_scale(const float *V, const float *wtab, int nw, cons
 
 hipError_t launch_encode(int fmt, const float *s, void *dst, float *sscale, int n12, int n3,
                          hipStream_t st) {
-  hipError_t e = hipMemsetAsync(sscale, 0, sizeof(float) * n3, st);
-  if (e != hipSuccess) return e;
+  // the per-slice maxima start at zero (a fill kernel, not a memset: the
+  // propagation loops that call this may be captured into a graph)
+  k_fill<<<grid_for(n3, 256), 256, 0, st>>>(sscale, 0.f, n3);
   // absmax: 16 values per lane per pass
   int ga = (n12 + 4095) / 4096;
   if (ga > 64) ga = 64;

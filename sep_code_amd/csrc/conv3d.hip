@@ -54,7 +54,14 @@ k_conv_fwd(const uint16_t *__restrict__ x, int xcs, const uint16_t *__restrict__
   constexpr int NCH = (KTOT + 31) / 32;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wt = Wd >> 4, hg = H / YT;
-  const long tile = (long)blockIdx.x * 4 + wave;
+  // XCD-aware order: workgroup b runs on XCD b % 8, so XCD k takes a
+  // contiguous run of tiles (x fastest, then rows, then z).  Neighbouring
+  // rows and planes, whose inputs the 27 taps share, then meet in that XCD's
+  // L2 instead of being fetched again by another XCD.
+  const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7;
+  const int bid = blockIdx.x, xcd = bid & 7, jj = bid >> 3;
+  const int blk = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const long tile = (long)blk * 4 + wave;
   if (tile >= (long)N * D * hg * wt) return;  // whole wave; no barriers below
   const int xt = (int)(tile % wt);
   long r = tile / wt;

@@ -181,6 +181,10 @@ class BornDevice:
         call("brtm_kernel_timing_result", self.h, C.byref(t), C.byref(n))
         return t.value, n.value
 
+    def set_graphs(self, mode=-1):
+        """Propagation loops as graphs: 1 on, 0 off, -1 automatic (small grids)."""
+        call("brtm_set_graphs", self.h, int(mode))
+
     def set_tuning(self, ty_tile=0, target_blocks=2048):
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 
