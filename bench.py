@@ -493,6 +493,14 @@ def main():
                    "grid": list(w.n), "nt": w.nt, "store": w.store, "ckpt": w.ckpt,
                    "graphs": graphs, "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "image_finite": finite,
+        # the same shot under other counts of cell-updates per time step:
+        # value credits the two wavefields of an RTM (source and receiver);
+        # one field per time step is n^3 nt / t; all propagated steps adds the
+        # checkpoint recompute (three fields per time step)
+        "rates_gpts": {"rtm_two_fields": round(value, 3),
+                       "one_field_per_time_step": round(value / 2, 3),
+                       "all_propagated_steps": round(value * (1.5 if w.store == "ckpt" else 1.0),
+                                                     3)},
     }
     dev.close()
     if a.lsrtm_iters > 0 and w.name == "rtm512":
