@@ -335,12 +335,13 @@ def lsrtm_main(a):
         log("cpu baseline ...")
         cpu = cpu_baseline(w, a.cpu_seconds)
         cpu.pop("extrapolated_shot", None)
-        cpu["sample"] += ("; one CG iteration advances 4 fields per shot (source field "
-                          "stored: Born forward, then the adjoint), 8 shots")
-        it_s = 4 * len(w.shots) * w.ncell * w.nt / (cpu["value"] * 1e9)
+        cpu["sample"] += ("; one CG iteration advances 2 fields per shot (the source "
+                          "field is stored: the scattered field in Born modelling, then the "
+                          "receiver field in the adjoint), 8 shots")
+        it_s = 2 * len(w.shots) * w.ncell * w.nt / (cpu["value"] * 1e9)
         cpu["extrapolated_cg_iteration_s"] = {
             "seconds": round(it_s, 1),
-            "basis": "extrapolated, not measured: the sampled per-step rate over 4 x 8 x "
+            "basis": "extrapolated, not measured: the sampled per-step rate over 2 x 8 x "
                      f"{w.nt} steps, imaging and the Born source not counted"}
     line = {
         "metric": "RTM Gpts/s (% HBM roofline); LSRTM time-to-1e-2 residual, learned vs plain CG",
