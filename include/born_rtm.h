@@ -217,6 +217,17 @@ int brtm_set_stream_loads(brtm_ctx *ctx, int mode);
  * No counterpart in the reference; the same results. */
 int brtm_set_graphs(brtm_ctx *ctx, int mode);
 
+/* Advance the source field two plain steps per pass (temporal blocking):
+ * the source propagation and the checkpoint recompute run steps (it, it+1),
+ * it even, as one kernel that reads p^n, p^{n-1} and V once and writes
+ * p^{n+1} and p^{n+2}, 20 bytes per cell for two steps instead of 32.  The
+ * source points are injected inside the pass, so it applies when they lie
+ * on one plane, one per cell (else the steps run one by one).  1 on, 0 off
+ * (the default), -1 automatic.  No counterpart in the reference; each step
+ * gives the same bits as the single step.  brtm_kernel_timing and
+ * brtm_time_step take form 2 for this pass. */
+int brtm_set_two_step(brtm_ctx *ctx, int mode);
+
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms
  * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE

@@ -70,6 +70,7 @@ SIGNATURES = {
     "brtm_set_stream_loads": [_P, _I],
     "brtm_set_born_path": [_P, _I],
     "brtm_set_graphs": [_P, _I],
+    "brtm_set_two_step": [_P, _I],
     "brtm_kernel_timing": [_P, _I, _I, _I, _I],
     "brtm_kernel_timing_result": [_P, C.POINTER(_D), C.POINTER(_I)],
     "brtm_copy_bench": [_P, _D, _I, C.POINTER(_D)],

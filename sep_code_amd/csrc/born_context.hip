@@ -97,6 +97,33 @@ constexpr int kAutoPrefetch = 1;
 // launches, which the hint would forfeit: 11 % slower at 256^3.
 constexpr double kStreamMinStepBytes = 512.0 * 1024 * 1024;
 
+#define BRTM_TRY(x)              \
+  do {                           \
+    int rc_ = (x);               \
+    if (rc_ != BRTM_OK) return rc_; \
+  } while (0)
+
+// Event pair around a timed launch (brtm_kernel_timing).
+static int ktime_begin(brtm_ctx *c, bool timed) {
+  if (!timed) return BRTM_OK;
+  if (c->ktime_used + 2 > c->ktime_ev.size()) {
+    for (int k = 0; k < 256; ++k) {
+      hipEvent_t e;
+      BRTM_HIP(hipEventCreate(&e));
+      c->ktime_ev.push_back(e);
+    }
+  }
+  BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
+  return BRTM_OK;
+}
+
+static int ktime_end(brtm_ctx *c, bool timed) {
+  if (!timed) return BRTM_OK;
+  BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
+  c->ktime_used += 2;
+  return BRTM_OK;
+}
+
 int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *po, float *pn,
          const void *snap, const float *sscale, float *img, const float *m, int rec, int it,
          int jt, const float *rscale, bool *fused) {
@@ -162,16 +189,7 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
   }
   const bool timed =
       c->ktime_on && form == c->ktime_form && act == c->ktime_act && fmt == c->ktime_fmt;
-  if (timed) {
-    if (c->ktime_used + 2 > c->ktime_ev.size()) {
-      for (int k = 0; k < 256; ++k) {
-        hipEvent_t e;
-        BRTM_HIP(hipEventCreate(&e));
-        c->ktime_ev.push_back(e);
-      }
-    }
-    BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used], c->stream));
-  }
+  BRTM_TRY(ktime_begin(c, timed));
   // receivers inside the step: the four-cell ring kernel, or the one-cell
   // kernel's 8-row small-grid shape; one receiver plane
   const bool src = rec == REC_SOURCE;
@@ -199,11 +217,68 @@ int step(brtm_ctx *c, int form, int act, int fmt, const float *p, const float *p
     BRTM_HIP(launch_step_x2(act, fmt, P, ty, c->stream));
   else
     BRTM_HIP(launch_step(form, act, fmt, P, ty, c->stream, kmode));
-  if (timed) {
-    BRTM_HIP(hipEventRecord(c->ktime_ev[c->ktime_used + 1], c->stream));
-    c->ktime_used += 2;
-  }
+  BRTM_TRY(ktime_end(c, timed));
   if (fused) *fused = in_step;
+  return BRTM_OK;
+}
+
+// Whether the source field's plain steps may run two per pass: the mode
+// allows it, n1 % 4 == 0, and the source points (if any) can be injected
+// inside the pass (one plane, one per cell).  Automatic: grids whose 64 x 32
+// tiles, cut into z chunks of at least 64 slices, still give every compute
+// unit a workgroup (the first step's 8 extra slices per chunk then cost at
+// most an eighth).
+static bool pair_ok(const brtm_ctx *c) {
+  if (c->two_step == 0 || c->n1 % 4 != 0) return false;
+  if (c->nsrc > 0 && !(c->src_plane && c->rec_fusion)) return false;
+  if (c->two_step < 0) {
+    const long tiles = (long)((c->n1 + 63) / 64) * ((c->n2 + 31) / 32);
+    if (tiles * (c->n3 - 8) / 64 < c->ncu) return false;
+  }
+  return true;
+}
+
+// Source-field steps it and it+1 in one pass: pn1 = s^{it+1} from
+// (p, po) = (s^it, s^{it-1}), pn2 = s^{it+2}, with the source points of steps
+// it and it+1 injected inside (Born/gpu_funcs_3d.cu:227-250 per step).
+// pn1 and pn2 are distinct from p and po and from each other.
+// src = false: the pass alone, without the source points (timing).
+static int step_pair(brtm_ctx *c, const float *p, const float *po, float *pn1, float *pn2,
+                     int it, int jt, bool src = true) {
+  StepParams P{};
+  P.n1 = c->n1;
+  P.n2 = c->n2;
+  P.n3 = c->n3;
+  P.nw = c->nw;
+  std::memcpy(P.c, c->coeffs, sizeof(P.c));
+  P.p = p;
+  P.po = po;
+  P.pn = pn1;
+  P.pn2 = pn2;
+  P.V = c->V.as<float>();
+  P.wtab = c->d_wtab.as<float>();
+  P.stream = c->stream_loads < 0 ? (16.0 * (double)c->ncell > kStreamMinStepBytes)
+                                 : c->stream_loads != 0;
+  int rec = REC_NONE;
+  if (src && c->nsrc > 0) {
+    BRTM_REQUIRE(c->src_plane && c->rec_fusion, BRTM_ESTATE,
+                 "two steps per pass need the source points on one plane");
+    rec = REC_INJECT;
+    P.rz = c->src_z;
+    P.nrec = c->nsrc;
+    P.rsp = c->ntr_s;
+    P.rst = 1;
+    P.rid = it / jt;
+    P.rid2 = (it + 1) / jt;
+    P.rmap = c->smap.as<int>();
+    P.rtr = c->strc.as<float>();
+    std::memcpy(P.rsinc, &c->sinc[0][8 * (it % jt)], sizeof(P.rsinc));
+    std::memcpy(P.rsinc2, &c->sinc[0][8 * ((it + 1) % jt)], sizeof(P.rsinc2));
+  }
+  const bool timed = c->ktime_on && c->ktime_form == 2;
+  BRTM_TRY(ktime_begin(c, timed));
+  BRTM_HIP(launch_step2(P, c->ncu, c->stream, rec));
+  BRTM_TRY(ktime_end(c, timed));
   return BRTM_OK;
 }
 
@@ -263,12 +338,6 @@ static int check_ready(brtm_ctx *c, int jt, int nt, int jimg) {
                BRTM_ESTATE, "interpolation tables have fewer than jt rows");
   return BRTM_OK;
 }
-
-#define BRTM_TRY(x)              \
-  do {                           \
-    int rc_ = (x);               \
-    if (rc_ != BRTM_OK) return rc_; \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // Graph launch of a propagation loop.  On small grids a step is short, and
@@ -364,6 +433,10 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     BRTM_TRY(S.data.alloc(2 * fbytes * S.count));
   }
   float *const wx = c->fb.as<float>(), *const wy = c->fa.as<float>();
+  // two more work buffers for steps run two per pass, whose outputs may not
+  // overwrite their inputs (the receiver-field buffers, idle here)
+  float *const wz = c->fc.as<float>(), *const ww = c->fd.as<float>();
+  const bool pairs = pair_ok(c) && !(kind == BRTM_STORE_CKPT && !ck_direct);
   float *cur = wx, *old = wy;  // s^it, s^{it-1}
   // Checkpoint pair K = (s^{K ckpt}, s^{K ckpt - 1}).  With ckpt >= 2 the
   // two steps before each checkpoint write straight into its pair, and the
@@ -391,9 +464,52 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
     BRTM_HIP(hipMemsetAsync(cur, 0, fbytes, c->stream));
     BRTM_HIP(hipMemsetAsync(old, 0, fbytes, c->stream));
   }
+  // the pass writes only the update set of its outputs
+  if (pairs) {
+    BRTM_HIP(launch_zero_halo(wz, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+    BRTM_HIP(launch_zero_halo(ww, c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+  }
   const int n12 = c->n1 * c->n2;
+  // where the output of step it must go (a checkpoint half or an FP32
+  // snapshot), or null: any free work buffer
+  auto fixed_out = [&](int it) -> float * {
+    if (ck_direct) {
+      if ((it + 1) % ckpt == 0 && (it + 1) / ckpt < S.count) return ck_cur((it + 1) / ckpt);
+      if ((it + 2) % ckpt == 0 && (it + 2) / ckpt < S.count)
+        return ck_cur((it + 2) / ckpt) + c->ncell;
+    } else if (fmt == FMT_F32 && it % jimg == 0) {
+      return S.data.as<float>() + (size_t)c->ncell * (it / jimg);
+    }
+    return nullptr;
+  };
+  auto free_work = [&](const float *a, const float *b, const float *d) -> float * {
+    for (float *w : {wx, wy, wz, ww})
+      if (w != a && w != b && w != d) return w;
+    return nullptr;
+  };
+  auto encode = [&](int it, const float *out) -> int {
+    if (fmt != FMT_NONE && fmt != FMT_F32 && it % jimg == 0) {
+      const int k = it / jimg;
+      BRTM_HIP(launch_encode(fmt, out, S.data.as<char>() + 2 * (size_t)c->ncell * k,
+                             S.scale.as<float>() + (size_t)c->n3 * k, n12, c->n3, c->stream));
+    }
+    return BRTM_OK;
+  };
   auto loop = [&]() -> int {
   for (int it = 0; it < nt; ++it) {
+    if (pairs && it % 2 == 0 && it + 1 < nt) {
+      float *o1 = fixed_out(it);
+      if (!o1) o1 = free_work(cur, old, nullptr);
+      float *o2 = fixed_out(it + 1);
+      if (!o2) o2 = free_work(cur, old, o1);
+      BRTM_TRY(step_pair(c, cur, old, o1, o2, it, jt));
+      BRTM_TRY(encode(it, o1));
+      BRTM_TRY(encode(it + 1, o2));
+      old = o1;
+      cur = o2;
+      ++it;
+      continue;
+    }
     if (kind == BRTM_STORE_CKPT && !ck_direct && it % ckpt == 0) {
       char *base = S.data.as<char>() + 2 * fbytes * (size_t)(it / ckpt);
       BRTM_HIP(hipMemcpyAsync(base, cur, fbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -405,24 +521,19 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
         out = ck_cur((it + 1) / ckpt);                 // s^{it+1} opens checkpoint K
       else if ((it + 2) % ckpt == 0 && (it + 2) / ckpt < S.count)
         out = ck_cur((it + 2) / ckpt) + c->ncell;      // s^{it+1} = s^{K ckpt - 1}
-      else if (old != wx && old != wy)
-        out = cur == wx ? wy : wx;                     // s^{it-1} is a checkpoint: keep it
+      else if (old != wx && old != wy && old != wz && old != ww)
+        out = free_work(cur, old, nullptr);            // s^{it-1} is a checkpoint: keep it
     } else if (fmt == FMT_F32) {
       if (it % jimg == 0)
         out = S.data.as<float>() + (size_t)c->ncell * (it / jimg);  // snapshot k itself
-      else if (old != wx && old != wy)
-        out = cur == wx ? wy : wx;                     // s^{it-1} is a snapshot: keep it
+      else if (old != wx && old != wy && old != wz && old != ww)
+        out = free_work(cur, old, nullptr);            // s^{it-1} is a snapshot: keep it
     }
     bool fused = false;
     BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, cur, old, out, nullptr, nullptr, nullptr,
                   nullptr, REC_SOURCE, it, jt, nullptr, &fused));
     if (!fused) BRTM_TRY(inject_source(c, it, jt, out));
-    if (fmt != FMT_NONE && it % jimg == 0) {
-      const int k = it / jimg;
-      if (fmt != FMT_F32)  // (FP32: the step wrote snapshot k)
-        BRTM_HIP(launch_encode(fmt, out, S.data.as<char>() + 2 * (size_t)c->ncell * k,
-                               S.scale.as<float>() + (size_t)c->n3 * k, n12, c->n3, c->stream));
-    }
+    BRTM_TRY(encode(it, out));  // (FP32: the step wrote snapshot k)
     old = cur;
     cur = out;
   }
@@ -567,6 +678,8 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
     c->seg_slots = ck;
   }
   auto slot = [&](int j) { return c->seg.as<float>() + (size_t)c->ncell * j; };
+  // the recompute pairs the steps as the source propagation did (it even)
+  const bool pairs = pair_ok(c);
   return run_loop(c, [&]() -> int {
   for (int cidx = S.count - 1; cidx >= 0; --cidx) {
     const int k0 = cidx * ck, k1 = std::min(k0 + ck, nt);
@@ -577,6 +690,11 @@ int rtm_adjoint(brtm_ctx *c, int jt, int nt, int jimg) {
       const int j = it - k0;
       const float *p = j == 0 ? ck_p : slot(j - 1);
       const float *po = j == 0 ? ck_po : (j == 1 ? ck_p : slot(j - 2));
+      if (pairs && it % 2 == 0 && it + 1 < k1) {
+        BRTM_TRY(step_pair(c, p, po, slot(j), slot(j + 1), it, jt));
+        ++it;
+        continue;
+      }
       bool fused = false;
       BRTM_TRY(step(c, FORM_VL, ACT_NONE, FMT_NONE, p, po, slot(j), nullptr, nullptr, nullptr,
                     nullptr, REC_SOURCE, it, jt, nullptr, &fused));
@@ -1035,7 +1153,9 @@ int brtm_propagate(brtm_ctx *c, int form, int nsteps, float *p_cur, float *p_old
 
 int brtm_time_step(brtm_ctx *c, int form, int act, int fmt, int nrep, double *avg_ms) {
   BRTM_REQUIRE(c && avg_ms && nrep >= 1, BRTM_EINVAL, "bad arguments");
-  BRTM_REQUIRE(form == 0 || form == 1, BRTM_EINVAL, "form must be 0 or 1");
+  BRTM_REQUIRE(form >= 0 && form <= 2, BRTM_EINVAL, "form must be 0, 1 or 2");
+  BRTM_REQUIRE(form != 2 || (act == ACT_NONE && c->n1 % 4 == 0), BRTM_EINVAL,
+               "the two-step pass is plain and needs n1 % 4 == 0");
   BRTM_REQUIRE(act >= ACT_NONE && act <= ACT_BORN, BRTM_EINVAL, "act must be 0..2");
   BRTM_REQUIRE(fmt >= FMT_NONE && fmt <= FMT_F16, BRTM_EINVAL, "fmt must be 0..3");
   BRTM_REQUIRE((act == ACT_NONE) == (fmt == FMT_NONE), BRTM_EINVAL,
@@ -1049,12 +1169,26 @@ int brtm_time_step(brtm_ctx *c, int form, int act, int fmt, int nrep, double *av
   const void *snap = c->fd.ptr;  // any resident field serves as the snapshot
   float *img = act == ACT_IMG ? c->img.as<float>() : nullptr;
   const float *m = act == ACT_BORN ? c->mdl.as<float>() : nullptr;
+  if (form == 2) {
+    // two steps per launch over four buffers: (p, po) -> (pn1, pn2)
+    float *b[4] = {cur, old, c->fc.as<float>(), c->fd.as<float>()};
+    BRTM_HIP(launch_zero_halo(b[2], c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+    BRTM_HIP(launch_zero_halo(b[3], c->ncell, 1, c->n1, c->n2, c->n3, c->stream));
+    BRTM_TRY(step_pair(c, b[0], b[1], b[2], b[3], 0, 1, false));
+    BRTM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nrep; ++s) {
+      BRTM_TRY(step_pair(c, b[3], b[2], b[1], b[0], 0, 1, false));
+      std::swap(b[0], b[3]);
+      std::swap(b[1], b[2]);
+    }
+  } else {
   // warm-up launch, then the timed run
   BRTM_TRY(step(c, form, act, fmt, cur, old, old, snap, sc.as<float>(), img, m));
   BRTM_HIP(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nrep; ++s) {
     BRTM_TRY(step(c, form, act, fmt, cur, old, old, snap, sc.as<float>(), img, m));
     std::swap(cur, old);
+  }
   }
   BRTM_HIP(hipEventRecord(c->ev1, c->stream));
   BRTM_HIP(hipEventSynchronize(c->ev1));
@@ -1181,6 +1315,12 @@ int brtm_sync(brtm_ctx *c) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
   BRTM_HIP(hipStreamSynchronize(c->stream));
   free_graphs(c);
+  return BRTM_OK;
+}
+
+int brtm_set_two_step(brtm_ctx *c, int mode) {
+  BRTM_REQUIRE(c && mode >= -1 && mode <= 1, BRTM_EINVAL, "mode must be -1, 0 or 1");
+  c->two_step = mode;
   return BRTM_OK;
 }
 

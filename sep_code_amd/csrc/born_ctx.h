@@ -95,6 +95,8 @@ struct brtm_ctx {
   int prefetch = 0;        // load lookahead of the plain four-cell ring step: 0 auto, 1, 2
   int stream_loads = -1;   // streaming loads in the four-cell ring step: -1 auto, 0, 1
   int graphs = -1;         // propagation loops launched as graphs: -1 auto, 0, 1
+  // plain source-field steps two per pass (launch_step2): -1 auto, 0, 1
+  int two_step = 0;
   // executable graphs launched and not yet known to be finished: freed once
   // the stream has been synchronised
   std::vector<hipGraphExec_t> graph_execs;

@@ -57,6 +57,11 @@ struct StepParams {
   float *rtr;
   const float *rscale;
   float rsinc[8];
+  // two-step pass (launch_step2): p^{n+1} goes to pn, p^{n+2} to pn2; the
+  // second step's receiver time index and taps
+  float *pn2;
+  int rid2;
+  float rsinc2[8];
 };
 
 // rec (RecMode): the receivers or source points of one plane inside the
@@ -72,6 +77,14 @@ hipError_t launch_step_x2(int act, int fmt, StepParams P, int ty_tile, hipStream
 // ring only) handles the receivers inside the step.
 hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring,
                           hipStream_t st, int rec = REC_NONE);
+// Two consecutive plain VL steps in one pass: p^{n+1} = step(p = p^n,
+// po = p^{n-1}) into pn and p^{n+2} = step(p^{n+1}, p^n) into pn2, with the
+// sponge, and (rec = REC_INJECT) the source points of plane rz injected after
+// each step (time indices rid, rid2; taps rsinc, rsinc2).  64 x 32 output
+// tiles; the first step also runs on the tile's 4-cell rim.  pn and pn2 must
+// not alias p, po or each other (other tiles read p and po around their
+// own).  n1 % 4 == 0.  Fills the tile geometry itself.
+hipError_t launch_step2(StepParams P, int target_blocks, hipStream_t st, int rec = REC_NONE);
 // Tile / chunk geometry chosen for a grid (fills ntx..zc of P); xw is the
 // tile width in x (64, 128 or 256 for the 1-, 2- and 4-cell kernels).
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw = 64);

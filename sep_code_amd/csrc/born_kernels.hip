@@ -22,6 +22,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 namespace brtm {
 
 namespace {
@@ -1218,6 +1220,308 @@ k_step_x4r(const StepParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Two time steps in one pass (temporal blocking of the plain VL step).
+//
+// A workgroup owns a 64 x 32 output tile of the x-y plane and streams z.  It
+// reads p^n on the tile plus an 8-cell rim, p^{n-1} and V on the tile plus a
+// 4-cell rim, and computes p^{n+1} = step(p^n, p^{n-1}) on that 72 x 40
+// region at plane z, then p^{n+2} = step(p^{n+1}, p^n) on the tile at plane
+// z - 4, where all nine z taps of p^{n+1} exist.  Both fields are written;
+// nothing of p^{n+1} goes through memory between the steps, so a pair of
+// steps moves 20 bytes per cell (read p^n, p^{n-1}, V; write p^{n+1},
+// p^{n+2}) instead of 32.
+//
+// Threads: 512, one 4-cell group of the output tile each (16 groups per row,
+// 32 rows).  Lanes 0..25 of each wave also own one group of the first
+// step's rim (whose first step they compute), lanes 26..55 one group of the
+// input rim (which they only load).  Each thread keeps p^n of its groups and V of
+// its tile group for ten planes in ten-slot register buffers; the z loop is
+// unrolled ten times, so every slot index is a compile-time constant and the
+// buffers turn without register moves (a chunk's iteration count is rounded
+// up to a multiple of ten; the extra iterations store nothing).  Plane z of
+// p^n goes from the registers into a two-plane LDS buffer over the 80 x 48
+// input region for the first step's x-y taps; p^{n+1} goes into a ten-plane
+// LDS ring over the 72 x 40 first-step region, which gives the second step
+// its x-y taps at z - 4 and its z taps at z - 8 .. z (30 + 113 KB).  One
+// barrier per iteration.  The source points' contributions are computed
+// once per launch for the thread's cells.
+//
+// The per-cell arithmetic is the four-cell ring step's, operation for
+// operation (centre, then for each distance the x, y and z pairs), so each
+// of the two steps gives the single step's bits.
+// ---------------------------------------------------------------------------
+template <int REC, bool STREAM>
+__global__ void __launch_bounds__(512)
+k_step2_x4(const StepParams P) {
+  constexpr int TXV = 16, TY = 32;           // output tile: 16 groups of 4 x 32 rows
+  constexpr int W1 = TXV + 2, H1 = TY + 8;   // first-step region, in groups / rows
+  constexpr int W0 = TXV + 4, H0 = TY + 16;  // input region
+  constexpr int NT = TXV * TY;
+  constexpr int NS = 10;  // register slots, p^{n+1} ring planes, loop unroll
+  constexpr int NRIM1 = W1 * H1 - TXV * TY;  // 208
+  constexpr int NRIM0 = W0 * H0 - W1 * H1;   // 240
+  __shared__ f32x4 s0[2][H0][W0];
+  __shared__ f32x4 s1[NS][H1][W1];
+  __shared__ float wsh[kWMax];
+
+  const int tid = threadIdx.x;
+  const int n1 = P.n1, n2 = P.n2, n3 = P.n3, nw = P.nw;
+  const size_t s3 = (size_t)n1 * n2;
+  for (int e = tid; e < nw; e += NT) wsh[e] = P.wtab[e];
+
+  const int nb = P.ntx * P.nty * P.ntz;
+  const int bid = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = bid & 7, jj = bid >> 3;
+  const int t = (xcd < r8) ? xcd * (q8 + 1) + jj : r8 * (q8 + 1) + (xcd - r8) * q8 + jj;
+  const int tx = t % P.ntx;
+  const int ty = (t / P.ntx) % P.nty;
+  const int tz = t / (P.ntx * P.nty);
+  const int x0 = tx * 4 * TXV, y0 = ty * TY;
+  const int z0 = 4 + tz * P.zc;
+  const int z1 = min(z0 + P.zc, n3 - 4);
+
+  // A: the thread's group of the output tile
+  const int ro = tid / TXV, co = tid % TXV;
+  const int xa = x0 + 4 * co, ya = y0 + ro;
+  // X: a group of the first step's rim (B: rows 0-3 and 36-39 of that
+  // region, then its left and right columns) or of the input rim (C: rows
+  // 0-3 and 44-47 of the input region, then its side columns)
+  // (every wave holds 26 rim groups of the first step and 30 of the input)
+  constexpr int NWV = NT / 64, PB = NRIM1 / NWV, PC = NRIM0 / NWV;
+  static_assert(PB * NWV == NRIM1 && PC * NWV == NRIM0 && PB + PC <= 64, "rim split");
+  const int wv = tid >> 6, ln = tid & 63;
+  const bool hasB = ln < PB;
+  const bool hasC = ln >= PB && ln < PB + PC;
+  int rx, cx, xx, yx;   // LDS position in the input region, grid position
+  int rb = 0, cb = 0;   // B: position in the first-step region
+  if (hasB) {
+    const int ib = wv * PB + ln;
+    if (ib < 8 * W1) {
+      const int rr = ib / W1;
+      rb = rr < 4 ? rr : TY + rr;
+      cb = ib % W1;
+    } else {
+      const int u = ib - 8 * W1;
+      rb = 4 + (u >> 1);
+      cb = (u & 1) ? W1 - 1 : 0;
+    }
+    rx = rb + 4;
+    cx = cb + 1;
+  } else {
+    const int uc = hasC ? wv * PC + ln - PB : 0;
+    if (uc < 8 * W0) {
+      const int rr = uc / W0;
+      rx = rr < 4 ? rr : TY + 8 + rr;
+      cx = uc % W0;
+    } else {
+      const int u = uc - 8 * W0;
+      rx = 4 + (u >> 1);
+      cx = (u & 1) ? W0 - 1 : 0;
+    }
+  }
+  xx = x0 - 8 + 4 * cx;
+  yx = y0 - 8 + rx;
+
+  // in the grid (may be loaded) / in the update set (computed and kept)
+  const bool hasX = hasB || hasC;
+  const bool mA = xa < n1 && ya < n2;
+  const bool mX = hasX && xx >= 0 && xx < n1 && yx >= 0 && yx < n2;
+  const bool uA = xa >= 4 && xa < n1 - 4 && ya >= 4 && ya < n2 - 4;
+  const bool uB = hasB && xx >= 4 && xx < n1 - 4 && yx >= 4 && yx < n2 - 4;
+  const int oA = mA ? ya * n1 + xa : 0;
+  const int oX = mX ? yx * n1 + xx : 0;
+  // the sponge's x-y edge distance per cell (clamped at 0: cells outside the
+  // update set are computed and discarded)
+  int exyA[4], exyB[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    exyA[j] = max(min(edist(xa + j, n1), edist(ya, n2)), 0);
+    exyB[j] = max(min(edist(xx + j, n1), edist(yx, n2)), 0);
+  }
+
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  auto ldp = [&](int z, int off, bool m) -> f32x4 {
+    f32x4 v = zero;
+    if (m && z >= 0 && z < n3) v = *reinterpret_cast<const f32x4 *>(P.p + (size_t)z * s3 + off);
+    return v;
+  };
+  // p^{n-1} and V: read by the first step of this tile only where the cell
+  // is in the update set
+  auto ldc = [&](const float *a, int z, int off, bool m) -> f32x4 {
+    f32x4 v = zero;
+    if (m && z >= 4 && z < n3 - 4) {
+      if constexpr (STREAM) v = ld4_stream(a + (size_t)z * s3, off);
+      else v = *reinterpret_cast<const f32x4 *>(a + (size_t)z * s3 + off);
+    }
+    return v;
+  };
+  auto put0 = [&](int z, const f32x4 &a, const f32x4 &x) {
+    f32x4(*pl)[W0] = s0[z % 2];
+    pl[ro + 8][co + 2] = a;
+    if (hasX) pl[rx][cx] = x;
+  };
+  // One VL update of the 4-cell group at region position (row, col) of the
+  // plane pl (width W); q[k] is the group's value at z offset k - 4; p_old
+  // po, V v, the sponge weight of edge distance min(exy, ez).
+  const float c0 = P.c[0];
+  auto update = [&](const f32x4 *pl, int W, int row, int col, const f32x4 *q, const f32x4 &po,
+                    const f32x4 &v, const int *exy, int ez) -> f32x4 {
+    const f32x4 C = q[4];
+    const f32x4 L = pl[row * W + col - 1];
+    const f32x4 R = pl[row * W + col + 1];
+    const float a[12] = {L.x, L.y, L.z, L.w, C.x, C.y, C.z, C.w, R.x, R.y, R.z, R.w};
+    float acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = c0 * a[4 + j];
+#pragma unroll
+    for (int k = 1; k <= 4; ++k) {
+      const f32x4 up = pl[(row - k) * W + col];
+      const f32x4 dn = pl[(row + k) * W + col];
+      const float c1 = P.c[1 + 3 * (k - 1)], c2 = P.c[2 + 3 * (k - 1)], c3 = P.c[3 + 3 * (k - 1)];
+      const f32x4 ud = up + dn;
+      const f32x4 zz = q[4 - k] + q[4 + k];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[j] = fmaf(c1, a[4 + j - k] + a[4 + j + k], acc[j]);
+        acc[j] = fmaf(c2, ud[j], acc[j]);
+        acc[j] = fmaf(c3, zz[j], acc[j]);
+      }
+    }
+    f32x4 pn;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = min(exy[j], ez);
+      const float w = e < nw ? wsh[e] : 1.f;
+      pn[j] = w * fmaf(v[j], acc[j], 2.f * C[j]) - (w * w) * po[j];
+    }
+    return pn;
+  };
+  // The source points of plane rz: k_inject's sum for the cells of a group
+  // at time index rid with taps rs, and which cells hold a point.
+  auto source_sum = [&](int x, int y, int rid, const float *rs, f32x4 &add, int &mask) {
+    const int4 rr = *reinterpret_cast<const int4 *>(P.rmap + y * n1 + x);
+    const int ri[4] = {rr.x, rr.y, rr.z, rr.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float acc = 0.f;
+      if (ri[j] >= 0) {
+        mask |= 1 << j;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (rs[k] != 0.f) acc = fmaf(rs[k], P.rtr[ri[j] * P.rsp + (rid + k) * P.rst], acc);
+      }
+      add[j] = acc;
+    }
+  };
+  f32x4 addA1 = zero, addB1 = zero, addA2 = zero;  // step 1 (tile, rim), step 2
+  int mskA = 0, mskB = 0;
+  if constexpr (REC == REC_INJECT) {
+    if (uA) {
+      source_sum(xa, ya, P.rid, P.rsinc, addA1, mskA);
+      int m2 = 0;
+      source_sum(xa, ya, P.rid2, P.rsinc2, addA2, m2);
+    }
+    if (uB) source_sum(xx, yx, P.rid, P.rsinc, addB1, mskB);
+  }
+  auto inject = [&](f32x4 &pn, const f32x4 &add, int mask) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (mask & (1 << j)) pn[j] += add[j];
+  };
+
+  // Iterations ts, ts+1, ...: the first step at plane z, the second at z - 4.
+  // Plane p's values sit in slot (p - ts) % NS of qa (p^n, tile group), qx
+  // (p^n, rim group) and vd (V, tile group), p_old and the rim group's V in
+  // slot (p - ts) % 2, and p^{n+1} of plane p in ring plane p % NS.
+  const int ts = z0 - 4, te = z1 + 4;
+  f32x4 qa[NS], qx[NS], vd[NS], poA[2], poB[2], vB[2];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) vd[k] = zero;
+  // planes ts-4 .. ts+4 (slots 6..9, 0..4)
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    qa[(k + 6) % NS] = ldp(ts - 4 + k, oA, mA);
+    qx[(k + 6) % NS] = ldp(ts - 4 + k, oX, mX);
+  }
+  poA[0] = ldc(P.po, ts, oA, uA);
+  poB[0] = ldc(P.po, ts, oX, uB);
+  vd[0] = ldc(P.V, ts, oA, uA);
+  vB[0] = ldc(P.V, ts, oX, uB);
+
+  auto iteration = [&](int z, auto uc) {
+    constexpr int u = decltype(uc)::value;
+    constexpr int cur = u % 2, nxt = (u + 1) % 2;
+    auto sl = [](int k) constexpr { return ((k % NS) + NS) % NS; };
+    put0(z, qa[sl(u)], qx[sl(u)]);
+    // loads for the next iteration (out-of-grid planes load nothing)
+    qa[sl(u + 5)] = ldp(z + 5, oA, mA);
+    qx[sl(u + 5)] = ldp(z + 5, oX, mX);
+    poA[nxt] = ldc(P.po, z + 1, oA, uA);
+    vd[sl(u + 1)] = ldc(P.V, z + 1, oA, uA);
+    poB[nxt] = ldc(P.po, z + 1, oX, uB);
+    vB[nxt] = ldc(P.V, z + 1, oX, uB);
+    __syncthreads();
+
+    // first step at plane z over the tile and its rim (computed everywhere,
+    // kept in the update set)
+    const bool zin = z >= 4 && z < n3 - 4;
+    const int ez = max(edist(z, n3), 0);
+    const f32x4 *pl0 = &s0[z % 2][0][0];
+    f32x4(*r1)[W1] = s1[z % NS];
+    f32x4 za[9], zx[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      za[k] = qa[sl(u - 4 + k)];
+      zx[k] = qx[sl(u - 4 + k)];
+    }
+    f32x4 pA = update(pl0, W0, ro + 8, co + 2, za, poA[cur], vd[sl(u)], exyA, ez);
+    if constexpr (REC == REC_INJECT)
+      if (z == P.rz) inject(pA, addA1, mskA);
+    if (!(zin && uA)) pA = zero;
+    r1[ro + 4][co + 1] = pA;
+    if (hasB) {
+      f32x4 pB = update(pl0, W0, rx, cx, zx, poB[cur], vB[cur], exyB, ez);
+      if constexpr (REC == REC_INJECT)
+        if (z == P.rz) inject(pB, addB1, mskB);
+      if (!(zin && uB)) pB = zero;
+      r1[rb][cb] = pB;
+    }
+
+    // second step at plane z - 4 over the tile: z taps of p^{n+1} at
+    // z - 8 .. z from the ring (plane z is the thread's own pA)
+    const int z2 = z - 4;
+    f32x4 z1q[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) z1q[k] = s1[(z + 2 + k) % NS][ro + 4][co + 1];
+    z1q[8] = pA;
+    const f32x4 *pl1 = &s1[(z + 6) % NS][0][0];
+    f32x4 p2 = update(pl1, W1, ro + 4, co + 1, z1q, qa[sl(u - 4)], vd[sl(u - 4)], exyA,
+                      max(edist(z2, n3), 0));
+    if (z2 >= z0 && z2 < z1 && uA) {
+      if constexpr (REC == REC_INJECT)
+        if (z2 == P.rz) inject(p2, addA2, mskA);
+      const size_t i = (size_t)z2 * s3 + oA;
+      __builtin_nontemporal_store(z1q[4], reinterpret_cast<f32x4 *>(P.pn + i));
+      __builtin_nontemporal_store(p2, reinterpret_cast<f32x4 *>(P.pn2 + i));
+    }
+  };
+
+  const int nit = ((te - ts + NS - 1) / NS) * NS;
+  for (int zt = ts; zt < ts + nit; zt += NS) {
+    iteration(zt + 0, std::integral_constant<int, 0>{});
+    iteration(zt + 1, std::integral_constant<int, 1>{});
+    iteration(zt + 2, std::integral_constant<int, 2>{});
+    iteration(zt + 3, std::integral_constant<int, 3>{});
+    iteration(zt + 4, std::integral_constant<int, 4>{});
+    iteration(zt + 5, std::integral_constant<int, 5>{});
+    iteration(zt + 6, std::integral_constant<int, 6>{});
+    iteration(zt + 7, std::integral_constant<int, 7>{});
+    iteration(zt + 8, std::integral_constant<int, 8>{});
+    iteration(zt + 9, std::integral_constant<int, 9>{});
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Injection / extraction (one thread per point).
 // ---------------------------------------------------------------------------
 __global__ void k_inject(const InjectParams P) {
@@ -1748,6 +2052,28 @@ hipError_t launch_step_x4(int act, int fmt, StepParams P, int ty_tile, bool ring
   }
   if (ty_tile == 8) return launch_x4_t<false, 8, 1>(act, fmt, P, st, rec);
   return launch_x4_t<false, 16, 2>(act, fmt, P, st, rec);
+}
+
+template <int REC>
+hipError_t launch_step2_r(const StepParams &P, hipStream_t st) {
+  const dim3 grid(P.ntx * P.nty * P.ntz), block(512);
+  if (P.stream) k_step2_x4<REC, true><<<grid, block, 0, st>>>(P);
+  else k_step2_x4<REC, false><<<grid, block, 0, st>>>(P);
+  return hipGetLastError();
+}
+
+hipError_t launch_step2(StepParams P, int target_blocks, hipStream_t st, int rec) {
+  if (P.nw > kWMax || P.n3 < 9 || P.n2 < 9 || P.n1 < 12 || (P.n1 & 3)) return hipErrorInvalidValue;
+  if ((long)P.n1 * P.n2 > 0x7fffffffL) return hipErrorInvalidValue;  // in-plane int offsets
+  if (!P.p || !P.po || !P.pn || !P.pn2 || !P.V || P.pn == P.p || P.pn == P.po ||
+      P.pn2 == P.p || P.pn2 == P.po || P.pn == P.pn2)
+    return hipErrorInvalidValue;
+  if (rec != REC_NONE &&
+      (rec != REC_INJECT || !P.rmap || !P.rtr || P.nrec <= 0 || P.rz < 4 || P.rz >= P.n3 - 4))
+    return hipErrorInvalidValue;
+  plan_step(P, 32, target_blocks, 64);
+  if (rec == REC_INJECT) return launch_step2_r<REC_INJECT>(P, st);
+  return launch_step2_r<REC_NONE>(P, st);
 }
 
 void plan_step(StepParams &P, int ty_tile, int target_blocks, int xw) {

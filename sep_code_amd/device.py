@@ -185,6 +185,11 @@ class BornDevice:
         """Propagation loops as graphs: 1 on, 0 off, -1 automatic (small grids)."""
         call("brtm_set_graphs", self.h, int(mode))
 
+    def set_two_step(self, mode=1):
+        """Source-field steps two per pass (temporal blocking): 1 on, 0 off
+        (the default), -1 automatic (large grids)."""
+        call("brtm_set_two_step", self.h, int(mode))
+
     def set_tuning(self, ty_tile=0, target_blocks=2048):
         call("brtm_set_tuning", self.h, int(ty_tile), int(target_blocks))
 

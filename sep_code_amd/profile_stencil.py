@@ -6,7 +6,8 @@ runs, on one context: a plain 16-byte streaming copy of --copy-gib GiB (the
 calibration kernel whose FETCH_SIZE / WRITE_SIZE per byte scale the other
 kernels' counters), then --reps launches each of the stencil variants
   VL plain step, VL + fused imaging (FP32 snapshot), VL + fused Born source
-  (BF16 snapshot), VL + fused imaging (BF16 snapshot)
+  (BF16 snapshot), VL + fused imaging (BF16 snapshot), and the pass of two
+  plain VL steps (vl_pair; its cells-per-lane and ring settings are fixed)
 and prints their HIP-event mean durations and algorithmic bandwidths, for each
 cells-per-lane setting in --xl (1: 64-wide tiles; 2: 128-wide tiles with
 8-byte accesses; 4: 256-wide tiles with 16-byte accesses; names get an
@@ -26,9 +27,10 @@ import numpy as np
 
 from .device import BornDevice
 
-# (name, form, act, fmt, algorithmic bytes per cell)
+# (name, form, act, fmt, algorithmic bytes per cell); form 2 is the pass of
+# two plain steps (read p, p_old, V; write two fields), "gpts" counting both
 VARIANTS = [("vl", 0, 0, 0, 16), ("vl_img_f32", 0, 1, 1, 28), ("vl_born_bf16", 0, 2, 2, 22),
-            ("vl_img_bf16", 0, 1, 2, 26)]
+            ("vl_img_bf16", 0, 1, 2, 26), ("vl_pair", 2, 0, 0, 20)]
 
 
 def main():
@@ -78,7 +80,7 @@ def main():
                                ({-1: "", 0: "_nostream", 1: "_stream"}[sl]) +
                                (f"_ty{a.ty}_tb{tb}" if a.ty else ""))
                         out[key] = {"ms": ms, "alg_GBps": B * n ** 3 / ms / 1e6,
-                                    "gpts": n ** 3 / ms / 1e6}
+                                    "gpts": (2 if form == 2 else 1) * n ** 3 / ms / 1e6}
     print(json.dumps(out), flush=True)
     dev.close()
 
