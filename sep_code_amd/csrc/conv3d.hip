@@ -2,11 +2,11 @@
 //
 // Forward (and input-gradient) convolutions are implicit GEMMs: M = output
 // channels, N = 16 consecutive x positions (of YT rows) per wave, K = the 27
-// taps x input channels flattened and cut in chunks of 32, one
-// v_mfma_f32_16x16x32_bf16 per (16-channel output tile, row, K chunk).
-// Operand lanes: lane l holds A[l%16][8(l/16)+i] and
-// B[8(l/16)+i][l%16] (i < 8), and receives D[4(l/16)+j][l%16] (j < 4)
-// (checked on the device against a host product).  In channels-last layout
+// taps x input channels, one v_mfma_f32_16x16x32_bf16 per (16-channel output
+// tile, row, tap, 32-channel chunk) and one v_mfma_f32_16x16x16_bf16 per
+// 16-channel remainder.  Operand lanes: lane l holds A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16] (i < 8; K = 16: 4(l/16)+i, i < 4), and receives
+// D[4(l/16)+j][l%16] (j < 4) (checked on the device against a host product).  In channels-last layout
 // the 8 K values of a lane are 8 consecutive channels of one position: one
 // 16-byte load.  The weight gradient is a GEMM over positions (K) staged
 // through LDS transposed so both operands are again 16-byte lane loads.
@@ -31,6 +31,14 @@ __device__ __forceinline__ bf16x8 ld8(const uint16_t *p) {
 }
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
 
+// four bf16 values (8 bytes) as the K = 16 MFMA's operand type
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ i16x4 ld4h(const uint16_t *p) {
+  const uint2 u = *reinterpret_cast<const uint2 *>(p);
+  return __builtin_bit_cast(i16x4, u);
+}
+__device__ __forceinline__ i16x4 zero4h() { return __builtin_bit_cast(i16x4, make_uint2(0, 0)); }
+
 __host__ __device__ inline int cpad(int c) { return (c + 31) / 32 * 32; }
 int grid1(long n, int bs = 256) {
   long g = (n + bs - 1) / bs;
@@ -38,20 +46,24 @@ int grid1(long n, int bs = 256) {
 }
 
 // ------------------------------------------------------------- forward GEMM
-// K runs over (tap, channel) pairs flattened, 32 per MFMA: a lane's 8 K
-// values are 8 consecutive channels of one tap (CIN is a multiple of 8), so
-// a 16-channel layer packs two taps into one MFMA instead of padding each tap
-// to 32.  A wave computes 16 x positions of YT consecutive rows: the weight
-// operand of a K chunk is loaded once for the YT row tiles.
+// A wave computes 16 x positions of YT consecutive rows at one z, all output
+// channels.  K runs tap by tap: for each (dz, dx) and 32-channel chunk the
+// wave loads the YT + 2 input rows y0-1 .. y0+YT once (a lane's 8 K values
+// are 8 consecutive channels of one position: one 16-byte load) and the
+// three dy taps use them, so an input row is loaded once for three MFMAs of
+// each row tile.  The weight operand of a tap is loaded once for the YT row
+// tiles.  A 16-channel remainder (CIN = 16, 48) runs on the K = 16 MFMA
+// (v_mfma_f32_16x16x16_bf16: 4 K values per lane, one 8-byte load) instead
+// of being padded to 32.
 template <int CIN, int COUT, int YT, bool RELU, bool OUTF32>
 __global__ void __launch_bounds__(256)
 k_conv_fwd(const uint16_t *__restrict__ x, int xcs, const uint16_t *__restrict__ W,
            const float *__restrict__ bias, void *y, int ycs, int N, int D, int H, int Wd) {
-  static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
   constexpr int CIN_P = (CIN + 31) / 32 * 32;
   constexpr int MT = COUT / 16;
-  constexpr int KTOT = 27 * CIN;
-  constexpr int NCH = (KTOT + 31) / 32;
+  constexpr int NC = CIN / 32;          // 32-channel chunks
+  constexpr bool R16 = CIN % 32 == 16;  // and a 16-channel remainder
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wt = Wd >> 4, hg = H / YT;
   // XCD-aware order: workgroup b runs on XCD b % 8, so XCD k takes a
@@ -76,45 +88,99 @@ k_conv_fwd(const uint16_t *__restrict__ x, int xcs, const uint16_t *__restrict__
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int t = 0; t < YT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int ch = 0; ch < NCH; ++ch) {
-    const int k = ch * 32 + 8 * kq;
-    const bool kin = k < KTOT;
-    const int tap = kin ? k / CIN : 0;
-    const int c = k - tap * CIN;
-    const int z2 = zz + tap / 9 - 1, x2 = x0 + col + tap % 3 - 1;
-    const int dy = (tap / 3) % 3 - 1;
-    const bool okzx = kin && z2 >= 0 && z2 < D && x2 >= 0 && x2 < Wd;
-    const long base = ((long)nn * D + z2) * H;
-    bf16x8 a[MT];
+  for (int dz = 0; dz < 3; ++dz) {
+    const int z2 = zz + dz - 1;
+    if (z2 < 0 || z2 >= D) continue;  // the same for the whole wave
+    const long zb = ((long)nn * D + z2) * H;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-      a[mt] = kin ? ld8(W + ((long)tap * COUT + mt * 16 + col) * CIN_P + c) : zero8();
+    for (int dx = 0; dx < 3; ++dx) {
+      const int x2 = x0 + col + dx - 1;
+      const bool okx = x2 >= 0 && x2 < Wd;
 #pragma unroll
-    for (int t = 0; t < YT; ++t) {
-      const int y2 = y0 + t + dy;
-      const bool ok = okzx && y2 >= 0 && y2 < H;
-      const bf16x8 bv = ok ? ld8(x + ((base + y2) * Wd + x2) * xcs + c) : zero8();
+      for (int cg = 0; cg < NC; ++cg) {
+        const int c = cg * 32 + 8 * kq;
+        bf16x8 bv[YT + 2];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv, acc[mt][t], 0, 0, 0);
+        for (int rr = 0; rr < YT + 2; ++rr) {
+          const int y2 = y0 + rr - 1;
+          bv[rr] = okx && y2 >= 0 && y2 < H ? ld8(x + ((zb + y2) * Wd + x2) * xcs + c) : zero8();
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const int tap = dz * 9 + dy * 3 + dx;
+          bf16x8 a[MT];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt)
+            a[mt] = ld8(W + ((long)tap * COUT + mt * 16 + col) * CIN_P + c);
+#pragma unroll
+          for (int t = 0; t < YT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+              acc[mt][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv[t + dy], acc[mt][t], 0, 0, 0);
+        }
+      }
+      if constexpr (R16) {
+        const int c = NC * 32 + 4 * kq;
+        i16x4 bv[YT + 2];
+#pragma unroll
+        for (int rr = 0; rr < YT + 2; ++rr) {
+          const int y2 = y0 + rr - 1;
+          bv[rr] = okx && y2 >= 0 && y2 < H ? ld4h(x + ((zb + y2) * Wd + x2) * xcs + c) : zero4h();
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const int tap = dz * 9 + dy * 3 + dx;
+          i16x4 a[MT];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt)
+            a[mt] = ld4h(W + ((long)tap * COUT + mt * 16 + col) * CIN_P + c);
+#pragma unroll
+          for (int t = 0; t < YT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+              acc[mt][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], bv[t + dy], acc[mt][t], 0, 0, 0);
+        }
+      }
     }
   }
+  // a lane's four channels 4kq .. 4kq+3 of a 16-channel tile are adjacent:
+  // one 8-byte (BF16) or 16-byte (FP32) store when the channel stride
+  // keeps them aligned
+  const bool vec = (ycs & 3) == 0;
 #pragma unroll
   for (int t = 0; t < YT; ++t) {
     const long opos = (((long)nn * D + zz) * H + y0 + t) * Wd + x0 + col;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
+      const int c0 = mt * 16 + 4 * kq;
+      float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int co = mt * 16 + 4 * kq + j;
-        float v = acc[mt][t][j];
-        if constexpr (OUTF32) {
-          static_cast<float *>(y)[opos * ycs + co] = v;
+        v[j] = acc[mt][t][j];
+        if constexpr (!OUTF32) {
+          v[j] += bias[c0 + j];
+          if (RELU) v[j] = fmaxf(v[j], 0.f);
+        }
+      }
+      if constexpr (OUTF32) {
+        float *o = static_cast<float *>(y) + opos * ycs + c0;
+        if (vec) {
+          *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
         } else {
-          v += bias[co];
-          if (RELU) v = fmaxf(v, 0.f);
-          static_cast<uint16_t *>(y)[opos * ycs + co] = f2bf(v);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) o[j] = v[j];
+        }
+      } else {
+        uint16_t *o = static_cast<uint16_t *>(y) + opos * ycs + c0;
+        if (vec) {
+          *reinterpret_cast<uint2 *>(o) =
+              make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                         (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) o[j] = f2bf(v[j]);
         }
       }
     }
@@ -537,11 +603,16 @@ hipError_t fwd_y(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, 
   return hipGetLastError();
 }
 
-// Four rows per wave where that still gives 1024 workgroups, else one.
+// Four rows per wave where that still gives 1024 workgroups, else one; eight
+// for the 16-output-channel layers (one output tile: the rows are the
+// wave's only independent MFMA chains).
 template <int CIN, int COUT>
 hipError_t fwd_t(const Act &x, const uint16_t *W, const float *b, Act y, Vol v, bool relu,
                  bool f32, hipStream_t st) {
   const long blocks4 = (long)v.n * v.d * (v.h / 4) * (v.w / 16) / 4;
+  if constexpr (COUT == 16) {
+    if (v.h % 8 == 0 && blocks4 >= 2048) return fwd_y<CIN, COUT, 8>(x, W, b, y, v, relu, f32, st);
+  }
   if (v.h % 4 == 0 && blocks4 >= 1024) return fwd_y<CIN, COUT, 4>(x, W, b, y, v, relu, f32, st);
   return fwd_y<CIN, COUT, 1>(x, W, b, y, v, relu, f32, st);
 }
@@ -573,7 +644,7 @@ hipError_t wgrad_t(const Act &x, const Act &dy, float *dW, Vol v, hipStream_t st
 
 hipError_t conv3d_fwd(const Act &x, const uint16_t *W, const float *bias, Act y, Vol v,
                       bool relu, bool out_f32, hipStream_t st) {
-  if (v.w % 16 || x.c % 8 || x.cs % 8 || y.c % 16) return hipErrorInvalidValue;
+  if (v.w % 16 || x.c % 16 || x.cs % 8 || y.c % 16) return hipErrorInvalidValue;
 #define X(CI, CO) \
   if (x.c == CI && y.c == CO) return fwd_t<CI, CO>(x, W, bias, y, v, relu, out_f32, st);
   BRTM_CONV_FWD_PAIRS(X)
