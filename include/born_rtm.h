@@ -234,6 +234,15 @@ int brtm_set_two_step(brtm_ctx *ctx, int mode);
  * scale the counters of the other kernels to HBM bytes. */
 int brtm_copy_bench(brtm_ctx *ctx, double bytes, int nrep, double *avg_ms);
 
+/* Calibration: nrep launches of out = sum of nin (1..3) input streams of
+ * `bytes` bytes each (16-byte vector loads and stores; nt_store 1: the
+ * store is nontemporal, as in the stencil steps); avg_ms receives the mean
+ * launch duration.  nin = 3 is the plain step's mix (read p, p_old, V;
+ * write p_new) without its arithmetic, halos or LDS.  No counterpart in the
+ * reference. */
+int brtm_stream_bench(brtm_ctx *ctx, int nin, double bytes, int nt_store, int nrep,
+                      double *avg_ms);
+
 /* FP64 inner product sum_i (double)a[i] * (double)b[i] of two device FP32
  * arrays of n elements (n >= 0), into the host double *out, on the
  * context's stream; the same kernels and the same bits as brtm_vec_dot

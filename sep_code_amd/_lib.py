@@ -74,6 +74,7 @@ SIGNATURES = {
     "brtm_kernel_timing": [_P, _I, _I, _I, _I],
     "brtm_kernel_timing_result": [_P, C.POINTER(_D), C.POINTER(_I)],
     "brtm_copy_bench": [_P, _D, _I, C.POINTER(_D)],
+    "brtm_stream_bench": [_P, _I, _D, _I, _I, C.POINTER(_D)],
     "brtm_dot_noalloc": [_P, _P, _P, C.c_long, C.POINTER(_D)],
     "brtm_sync": [_P],
     "brtm_device_bytes": [_P, C.POINTER(_D)],

@@ -1263,6 +1263,33 @@ int brtm_copy_bench(brtm_ctx *c, double bytes, int nrep, double *avg_ms) {
   return BRTM_OK;
 }
 
+int brtm_stream_bench(brtm_ctx *c, int nin, double bytes, int nt_store, int nrep,
+                      double *avg_ms) {
+  BRTM_REQUIRE(c && avg_ms && nrep >= 1 && bytes >= 16 && nin >= 1 && nin <= 3, BRTM_EINVAL,
+               "bad arguments");
+  BRTM_HIP(hipSetDevice(c->device));
+  const long n16 = (long)(bytes / 16);
+  DevBuf in[3], out;
+  for (int k = 0; k < nin; ++k) {
+    BRTM_TRY(in[k].alloc(16 * (size_t)n16));
+    BRTM_HIP(hipMemsetAsync(in[k].ptr, 0, 16 * (size_t)n16, c->stream));
+  }
+  BRTM_TRY(out.alloc(16 * (size_t)n16));
+  auto run = [&]() -> hipError_t {
+    return launch_sum16(nin, nt_store != 0, in[0].ptr, in[nin > 1 ? 1 : 0].ptr,
+                        in[nin > 2 ? 2 : 0].ptr, out.ptr, n16, c->stream);
+  };
+  BRTM_HIP(run());  // warm-up
+  BRTM_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < nrep; ++r) BRTM_HIP(run());
+  BRTM_HIP(hipEventRecord(c->ev1, c->stream));
+  BRTM_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  BRTM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *avg_ms = ms / nrep;
+  return BRTM_OK;
+}
+
 int brtm_set_born_path(brtm_ctx *c, int direct_lv) {
   BRTM_REQUIRE(c, BRTM_EINVAL, "ctx is NULL");
   c->born_direct_lv = direct_lv != 0;

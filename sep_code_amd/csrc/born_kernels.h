@@ -161,6 +161,11 @@ hipError_t launch_data_to_trace(const float *data, float *tr, int nrec, int ntb,
                                 float inv_dt, hipStream_t st);
 // Streaming copy of n16 16-byte words (calibration of bandwidth counters).
 hipError_t launch_copy16(const void *src, void *dst, long n16, hipStream_t st);
+// out = a (+ b (+ c)): nin (1..3) streams of n16 16-byte words read, one
+// written (nontemporal store when nt_store); bandwidth calibration for a
+// kernel's read/write mix.
+hipError_t launch_sum16(int nin, bool nt_store, const void *a, const void *b, const void *c,
+                        void *out, long n16, hipStream_t st);
 // Plain transpose [rows][cols] -> [cols][rows].
 hipError_t launch_transpose(const float *in, float *out, long rows, long cols,
                             hipStream_t st);

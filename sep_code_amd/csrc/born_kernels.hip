@@ -1838,6 +1838,20 @@ __global__ void k_copy16(const uint4 *__restrict__ src, uint4 *dst, long n) {
   if (i < n) dst[i] = src[i];
 }
 
+// out = sum of NIN input streams, 16 bytes per thread per stream (the
+// stencil step's read/write mix for NIN = 3); NT: nontemporal store.
+template <int NIN, bool NT>
+__global__ void k_sum16(const f32x4 *__restrict__ a, const f32x4 *__restrict__ b,
+                        const f32x4 *__restrict__ c, f32x4 *out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f32x4 v = a[i];
+  if constexpr (NIN >= 2) v += b[i];
+  if constexpr (NIN >= 3) v += c[i];
+  if constexpr (NT) __builtin_nontemporal_store(v, out + i);
+  else out[i] = v;
+}
+
 __global__ void k_transpose(const float *__restrict__ in, float *out, long rows, long cols) {
   __shared__ float tile[32][33];
   const long c0 = (long)blockIdx.x * 32, r0 = (long)blockIdx.y * 32;
@@ -2236,6 +2250,24 @@ hipError_t launch_copy16(const void *src, void *dst, long n16, hipStream_t st) {
   if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
   k_copy16<<<dim3((unsigned)blocks), 256, 0, st>>>(static_cast<const uint4 *>(src),
                                                    static_cast<uint4 *>(dst), n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum16(int nin, bool nt_store, const void *a, const void *b, const void *c,
+                        void *out, long n16, hipStream_t st) {
+  const long blocks = (n16 + 255) / 256;
+  if (blocks > 0x7fffffffL || nin < 1 || nin > 3) return hipErrorInvalidValue;
+  const dim3 g((unsigned)blocks), t(256);
+  const f32x4 *A = static_cast<const f32x4 *>(a), *B = static_cast<const f32x4 *>(b),
+              *C = static_cast<const f32x4 *>(c);
+  f32x4 *O = static_cast<f32x4 *>(out);
+#define SUM16(N)                                                      \
+  if (nin == N) {                                                     \
+    if (nt_store) k_sum16<N, true><<<g, t, 0, st>>>(A, B, C, O, n16);  \
+    else k_sum16<N, false><<<g, t, 0, st>>>(A, B, C, O, n16);          \
+  }
+  SUM16(1) SUM16(2) SUM16(3)
+#undef SUM16
   return hipGetLastError();
 }
 

@@ -169,6 +169,13 @@ class BornDevice:
         call("brtm_copy_bench", self.h, float(nbytes), int(nrep), C.byref(ms))
         return ms.value
 
+    def stream_bench(self, nin, nbytes, nt_store=True, nrep=10):
+        """Mean ms of out = sum of nin input streams of nbytes each."""
+        ms = C.c_double()
+        call("brtm_stream_bench", self.h, int(nin), float(nbytes), int(bool(nt_store)), int(nrep),
+             C.byref(ms))
+        return ms.value
+
     def set_born_path(self, direct_lv):
         call("brtm_set_born_path", self.h, int(bool(direct_lv)))
 

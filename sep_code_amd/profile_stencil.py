@@ -4,7 +4,9 @@
 
 runs, on one context: a plain 16-byte streaming copy of --copy-gib GiB (the
 calibration kernel whose FETCH_SIZE / WRITE_SIZE per byte scale the other
-kernels' counters), then --reps launches each of the stencil variants
+kernels' counters), a kernel with the plain step's read/write mix (three
+n^3 FP32 streams read and summed into a fourth, with a plain and with a
+nontemporal store), then --reps launches each of the stencil variants
   VL plain step, VL + fused imaging (FP32 snapshot), VL + fused Born source
   (BF16 snapshot), VL + fused imaging (BF16 snapshot), and the pass of two
   plain VL steps (vl_pair; its cells-per-lane and ring settings are fixed)
@@ -60,6 +62,13 @@ def main():
     nbytes = a.copy_gib * 2 ** 30
     ms = dev.copy_bench(nbytes, a.reps)
     out["copy"] = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / ms / 1e6}
+    # the plain step's mix without its arithmetic: three n^3 FP32 streams
+    # read, one written (plain and nontemporal store)
+    fb = 4.0 * n ** 3
+    for nt in (0, 1):
+        ms = dev.stream_bench(3, fb, nt_store=bool(nt), nrep=a.reps)
+        out[f"read3_write1{'_ntstore' if nt else ''}"] = {"bytes": 4 * fb, "ms": ms,
+                                                          "GBps": 4 * fb / ms / 1e6}
     for ring in [int(v) for v in a.ring.split(",")]:
         dev.set_halo_ring(ring)
         for xl in [int(v) for v in a.xl.split(",")]:
