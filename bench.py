@@ -292,6 +292,7 @@ def lsrtm_slice(iters, log):
     net.load(DEFAULT_WEIGHTS)
     net.set_odd(True)
     net.set_guard(0.8, 0.05)
+    net.set_guard_mode(1)
     t0 = time.perf_counter()
     W = lsrtm.illumination_weight(op, 0.01, power=0.5)
     tw = time.perf_counter() - t0
@@ -299,7 +300,7 @@ def lsrtm_slice(iters, log):
     r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=iters, rel_tol=0.0, name="fpcg", noise=e,
                            precond=pre, history=4)
     m.close()
-    res["fpcg"] = rec(r, {"illumination_s": round(tw, 3), "history": 4,
+    res["fpcg"] = rec(r, {"illumination_s": round(tw, 3), "history": 4, "guard_mode": 1,
                           "guard_diagonal_steps": net.guard_count()})
     log(f"lsrtm learned FCG: {iters} it, {r.seconds / iters:.3f} s/it, |Lm-d|/|d| "
         f"{r.data_rel_resid_recomputed:.4f}, noise-free misfit {r.rel_resid_recomputed:.4f}")

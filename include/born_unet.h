@@ -74,6 +74,14 @@ int brtm_unet_set_split_weight(brtm_unet *net, const brtm_vec *w);
  * returns how many calls took W^2 g since the split weight was set. */
 int brtm_unet_set_guard(brtm_unet *net, double repeat_cos, double min_cos);
 int brtm_unet_guard_count(const brtm_unet *net);
+/* Guard mode 1 (the default): z = W^2 g + c, c = the network's correction
+ * (its split output minus W^2 g); when c repeats the previous correction
+ * (cos above the repeat threshold) only its component orthogonal to that one
+ * is kept, and a z that is no descent direction or nearly orthogonal to g
+ * falls back to W^2 g.  brtm_unet_guard_count counts both.  Mode 0 replaces
+ * a repeating network output by W^2 g outright.  No counterpart in the
+ * reference. */
+int brtm_unet_set_guard_mode(brtm_unet *net, int mode);
 /* The preconditioner uses the odd part of the network,
  * P(x) = (net(x) - net(-x)) / 2, on = 1 (the default for batch >= 2): the
  * ideal preconditioner is linear, hence odd, and the even part -- a

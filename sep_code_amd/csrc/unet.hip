@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <utility>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -137,8 +138,10 @@ struct brtm_unet {
   const brtm_vec *split_w = nullptr;  // split weight W of the preconditioner (not owned)
   brtm_vec *split_tmp = nullptr;      // W . g
   brtm_vec *znet_prev = nullptr;      // the network's previous output (guard)
+  brtm_vec *split_d = nullptr;        // W^2 g (guard mode 1)
   bool have_prev = false;
   double guard_repeat = 0.8, guard_min = 0.05;
+  int guard_mode = 1;                 // 1: W^2 g + new part of the correction; 0: repeat -> W^2 g
   int guard_uses = 0;                 // iterations that took W^2 g since the weight was set
   bool odd = true;                    // preconditioner uses the odd part of the network
   float *prm() const { return params.as<float>(); }
@@ -385,6 +388,7 @@ int brtm_unet_destroy(brtm_unet *u) {
   (void)hipStreamSynchronize(u->ctx->stream);
   brtm_vec_destroy(u->split_tmp);
   brtm_vec_destroy(u->znet_prev);
+  brtm_vec_destroy(u->split_d);
   delete u;
   return BRTM_OK;
 }
@@ -542,9 +546,11 @@ int brtm_unet_set_split_weight(brtm_unet *u, const brtm_vec *w) {
   if (w && (!u->split_tmp || brtm_vec_size(u->split_tmp) != brtm_vec_size(w))) {
     brtm_vec_destroy(u->split_tmp);
     brtm_vec_destroy(u->znet_prev);
-    u->split_tmp = u->znet_prev = nullptr;
+    brtm_vec_destroy(u->split_d);
+    u->split_tmp = u->znet_prev = u->split_d = nullptr;
     U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->split_tmp));
     U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->znet_prev));
+    U_TRY(brtm_vec_create(u->ctx, brtm_vec_size(w), &u->split_d));
   }
   u->split_w = w;
   u->have_prev = false;
@@ -562,6 +568,13 @@ int brtm_unet_set_guard(brtm_unet *u, double repeat_cos, double min_cos) {
 
 int brtm_unet_guard_count(const brtm_unet *u) { return u ? u->guard_uses : 0; }
 
+int brtm_unet_set_guard_mode(brtm_unet *u, int mode) {
+  BRTM_REQUIRE(u && (mode == 0 || mode == 1), BRTM_EINVAL, "guard mode must be 0 or 1");
+  u->guard_mode = mode;
+  u->have_prev = false;
+  return BRTM_OK;
+}
+
 int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
   brtm_unet *u = static_cast<brtm_unet *>(user);
   if (!u || u->gm1 == 0) {
@@ -577,6 +590,37 @@ int brtm_unet_precond(void *user, const brtm_vec *g, brtm_vec *z) {
     U_TRY(brtm_vec_mult(u->split_tmp, u->split_w));
     U_TRY(apply_signed(u, u->split_tmp, z, u->gm1, u->gm2, u->gm3, u->gov, -1.f, u->odd));
     U_TRY(brtm_vec_mult(z, u->split_w));
+    if (u->guard_mode == 1) {
+      // z = W^2 g + c with c the network's correction z_net - W^2 g; when c
+      // nearly repeats the previous correction, only its component
+      // orthogonal to that one is kept
+      brtm_vec *d = u->split_d;
+      U_TRY(brtm_vec_copy(d, u->split_tmp));
+      U_TRY(brtm_vec_mult(d, u->split_w));           // W^2 g
+      U_TRY(brtm_vec_scale_add(z, 1.0, d, -1.0));    // c
+      double cc = 0, cp = 0, pp = 0;
+      U_TRY(brtm_vec_dot(z, z, &cc));
+      bool rep = false;
+      if (u->have_prev) {
+        U_TRY(brtm_vec_dot(z, u->znet_prev, &cp));
+        U_TRY(brtm_vec_dot(u->znet_prev, u->znet_prev, &pp));
+        rep = pp > 0 && cc > 0 && cp > u->guard_repeat * std::sqrt(cc * pp);
+      }
+      U_TRY(brtm_vec_copy(u->split_tmp, z));         // c as produced, for the next call
+      if (rep) U_TRY(brtm_vec_scale_add(z, 1.0, u->znet_prev, -cp / pp));
+      std::swap(u->split_tmp, u->znet_prev);
+      u->have_prev = true;
+      U_TRY(brtm_vec_add(z, d));
+      double zg = 0, zz = 0, gg = 0;
+      U_TRY(brtm_vec_dot(z, g, &zg));
+      U_TRY(brtm_vec_dot(z, z, &zz));
+      U_TRY(brtm_vec_dot(g, g, &gg));
+      const bool bad = !(zg > 0) || !std::isfinite(zg) || !std::isfinite(zz) ||
+                       zg < u->guard_min * std::sqrt(zz * gg);
+      if (rep || bad) ++u->guard_uses;
+      if (bad) U_TRY(brtm_vec_copy(z, d));
+      return BRTM_OK;
+    }
     // Guard: use the diagonal W^2 g (what the split gives for an identity
     // network) instead of the network's direction when that direction
     //  * is not a descent direction or not finite;

@@ -49,7 +49,7 @@ def _entry(r, extra=None):
 
 def compare(dev, w, op, d, e, floor, *, niter=400, tol=1e-2, weights=DEFAULT_WEIGHTS, batch=8,
             even=False, guard="0.8,0.05", history=4, diag_eps="", split_eps=0.01,
-            skip_plain=False, quiet=True, log=None):
+            skip_plain=False, quiet=True, log=None, guard_mode=1):
     """Plain CG, optionally illumination-diagonal PCG for each eps in the
     comma list diag_eps, and learned flexible CG with the network in
     `weights` ('' skips it), each from m = 0 on the prepared operator op
@@ -85,6 +85,7 @@ def compare(dev, w, op, d, e, floor, *, niter=400, tol=1e-2, weights=DEFAULT_WEI
         net.set_odd(not even)
         rep, mn = (float(v) for v in guard.split(","))
         net.set_guard(rep, mn)
+        net.set_guard_mode(guard_mode)
         dims = lsrtm.model_box(w)[:3]
         t0 = time.perf_counter()
         W = lsrtm.illumination_weight(op, split_eps, power=0.5) if split_eps > 0 else None
@@ -97,7 +98,7 @@ def compare(dev, w, op, d, e, floor, *, niter=400, tol=1e-2, weights=DEFAULT_WEI
                                  "seconds": r.seconds + tw, "illumination_seconds": tw,
                                  "fallbacks": r.fallbacks,
                                  "guard_diagonal_steps": net.guard_count(), "odd": not even,
-                                 "guard": guard})
+                                 "guard": guard, "guard_mode": guard_mode})
         log(f"learned FPCG: {r.iterations} it, {r.seconds + tw:.1f}s, "
             f"judged {r.rel_resid_recomputed:.4e}, fallbacks {r.fallbacks}, "
             f"guard {net.guard_count()}")
@@ -122,6 +123,10 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--even", action="store_true",
                     help="use the whole network, not its odd part")
+    ap.add_argument("--guard-mode", type=int, default=1,
+                    help="1: the diagonal's direction plus the new part of the network's "
+                         "correction; 0: a repeating network direction is replaced by the "
+                         "diagonal's")
     ap.add_argument("--guard", default="0.8,0.05",
                     help="repeat_cos,min_cos of the split preconditioner's guard")
     ap.add_argument("--quiet", action="store_true",
@@ -172,7 +177,7 @@ def main():
     out = compare(dev, w, op, d, e, floor, niter=a.niter, tol=a.tol, weights=a.weights,
                   batch=a.batch, even=a.even, guard=a.guard, history=a.history,
                   diag_eps=a.diag_eps, split_eps=a.split_eps, skip_plain=a.skip_plain,
-                  quiet=a.quiet, log=log)
+                  quiet=a.quiet, log=log, guard_mode=a.guard_mode)
     print(json.dumps(out), flush=True)
     dev.close()
 

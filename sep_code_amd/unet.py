@@ -34,6 +34,7 @@ _SIGS = {
     "brtm_unet_set_odd": [_P, _I],
     "brtm_unet_set_guard": [_P, _D, _D],
     "brtm_unet_guard_count": [_P],
+    "brtm_unet_set_guard_mode": [_P, _I],
 }
 _registered = False
 
@@ -138,6 +139,11 @@ class UNet:
     def set_guard(self, repeat_cos=0.8, min_cos=0.05):
         """Thresholds of the split preconditioner's fall-back to W^2 g."""
         call("brtm_unet_set_guard", self.h, float(repeat_cos), float(min_cos))
+
+    def set_guard_mode(self, mode: int):
+        """1 (the default): W^2 g plus the new part of the network's
+        correction; 0: a repeating network direction is replaced by W^2 g."""
+        call("brtm_unet_set_guard_mode", self.h, int(mode))
 
     def guard_count(self) -> int:
         return int(_reg().brtm_unet_guard_count(self.h))

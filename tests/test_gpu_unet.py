@@ -156,9 +156,10 @@ def test_split_preconditioner_with_identity_network_is_the_diagonal(net):
 
 
 def test_split_guard_replaces_a_fixed_direction(net):
-    """A network whose output ignores its input (whole network, head weights
-    zero, a large head bias: net(x) = x + b) gives nearly the same direction
-    on every call; the guard then returns the diagonal direction W^2 g."""
+    """Guard mode 0: a network whose output ignores its input (whole
+    network, head weights zero, a large head bias: net(x) = x + b) gives
+    nearly the same direction on every call; the guard then returns the
+    diagonal direction W^2 g."""
     from sep_code_amd.operator import PRECOND_FN, DeviceVector
     from oracle import unet_oracle as U
     theta = _theta(net, head=0.0)
@@ -167,6 +168,7 @@ def test_split_guard_replaces_a_fixed_direction(net):
     theta[o] = 50.0
     net.set_params(theta)
     net.set_odd(False)
+    net.set_guard_mode(0)
     dims = (40, 36, 30)
     r = np.random.default_rng(9)
     W = r.uniform(0.2, 1.0, int(np.prod(dims))).astype(np.float32)
@@ -186,5 +188,73 @@ def test_split_guard_replaces_a_fixed_direction(net):
         want = W.astype(np.float64) ** 2 * g
         np.testing.assert_allclose(z, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
     finally:
+        net.set_guard_mode(1)
+        net.set_odd(True)
+        net.set_params(_theta(net))
+
+
+def test_split_guard_keeps_the_new_part_of_the_correction(net):
+    """Guard mode 1 against its rule restated in numpy.  The network's raw
+    split outputs z_k come from the same calls with the guard's thresholds
+    out of reach (mode 0, repeat cos 1, min cos 0).  With c_k = z_k - W^2 g_k
+    (the network's correction), mode 1 returns W^2 g_k + c_k, less the
+    projection of c_k on c_{k-1} when their cos exceeds 0.8, and W^2 g_k when
+    that is no descent direction or has cos < 0.05 with g_k.  The network
+    here has a large fixed head bias (a repeating part) and a head that
+    follows its input (a new part)."""
+    from sep_code_amd.operator import PRECOND_FN, DeviceVector
+    from oracle import unet_oracle as U
+    theta = _theta(net, head=0.5, seed=4)
+    lay, _ = U.layout()
+    o, _ = lay["bh"]
+    theta[o] = 10.0
+    net.set_params(theta)
+    net.set_odd(False)
+    dims = (40, 36, 30)
+    r = np.random.default_rng(11)
+    W = r.uniform(0.2, 1.0, int(np.prod(dims))).astype(np.float32)
+    gs = [r.standard_normal(W.size).astype(np.float32) for _ in range(4)]
+    wv = DeviceVector.from_array(net.dev, W)
+    zv = DeviceVector(net.dev, W.size)
+
+    def run(mode, rep, mn):
+        net.set_guard(rep, mn)
+        net.set_guard_mode(mode)
+        _, addr, user = net.precond(dims, overlap=8, split_weight=wv)
+        fn = PRECOND_FN(addr)
+        out = []
+        for g in gs:
+            gv = DeviceVector.from_array(net.dev, g)
+            assert fn(user, gv.h, zv.h) == 0
+            out.append(zv.numpy().astype(np.float64))
+        return out
+
+    try:
+        raw = run(0, 1.0, 0.0)
+        assert net.guard_count() == 0
+        got = run(1, 0.8, 0.05)
+        Wd = W.astype(np.float64)
+        prev = None
+        projected = 0
+        for g, zr, z in zip(gs, raw, got):
+            d = Wd * Wd * g
+            c = zr - d
+            cn = c.copy()
+            if prev is not None:
+                cp, pp, cc = c @ prev, prev @ prev, c @ c
+                if cp > 0.8 * np.sqrt(cc * pp):
+                    cn = c - (cp / pp) * prev
+                    projected += 1
+            prev = c
+            want = d + cn
+            zg = want @ g
+            if not zg > 0 or zg < 0.05 * np.sqrt((want @ want) * (g @ g)):
+                want = d
+            err = np.linalg.norm(z - want) / np.linalg.norm(want)
+            assert err < 1e-5, err
+        assert projected >= 1  # the fixed bias makes the corrections repeat
+        assert net.guard_count() >= projected
+    finally:
+        net.set_guard(0.8, 0.05)
         net.set_odd(True)
         net.set_params(_theta(net))
