@@ -398,6 +398,7 @@ int source_prop(brtm_ctx *c, int jt, int nt, int jimg, int kind, int ckpt, float
   BRTM_REQUIRE(c->nsrc == 0 || (nt - 1) / jt + 8 <= c->ntr_s, BRTM_EINVAL,
                "source traces shorter than (nt-1)/jt + 8 samples");
   Store &S = cur_store(c);
+  S.owner = nullptr;  // a caller that prepares it for an operator claims it afterwards
   const size_t fbytes = sizeof(float) * (size_t)c->ncell;
   const int fmt = fmt_of_store(kind);
   // Checkpoints written straight by the steps (ckpt >= 2) need zero halos:

@@ -59,10 +59,15 @@ struct Store {
   // steps writing into the slots never touch it), so a later propagation of
   // the same kind can reuse the buffer as it is
   bool halo_zero = false;
+  // the operator (brtm_op) whose brtm_op_prepare made this store, or null:
+  // operators on one context share its per-shot stores, and one releases or
+  // reads only the stores it made
+  const void *owner = nullptr;
   void release() {
     data.release();
     scale.release();
     halo_zero = false;
+    owner = nullptr;
   }
 };
 

@@ -100,6 +100,21 @@ void levinson(int m, const double *r, const double *g, double *f) {
   } while (0)
 
 // Shot geometry into the context (source + receivers, zero traces).
+// Whether shot s's stored source wavefield on the context is the one this
+// operator's brtm_op_prepare made.
+bool owns_store(const brtm_op *op, int s) {
+  const brtm_ctx *c = op->ctx;
+  return s < (int)c->stores.size() && c->stores[s].owner == op;
+}
+
+int check_store(const brtm_op *op, int s) {
+  if (owns_store(op, s)) return BRTM_OK;
+  set_error("the stored source wavefield of shot " + std::to_string(s) +
+            " was made by another operator on this context (or released): call "
+            "brtm_op_prepare again");
+  return BRTM_ESTATE;
+}
+
 int load_shot(brtm_op *op, int s) {
   brtm_ctx *c = op->ctx;
   const brtm_op_desc &d = op->d;
@@ -296,9 +311,11 @@ int brtm_op_destroy(brtm_op *op) {
   if (!op) return BRTM_OK;
   (void)hipSetDevice(op->ctx->device);
   (void)hipStreamSynchronize(op->ctx->stream);
-  // a destroyed context has already freed the shots' stores
+  // a destroyed context has already freed the shots' stores; stores that
+  // another operator on the context has prepared since are not this one's
   if (!op->ctx->destroyed)
-    for (int s = 0; s < op->d.nshots; ++s) brtm_release_store(op->ctx, s);
+    for (int s = 0; s < op->d.nshots; ++s)
+      if (owns_store(op, s)) brtm_release_store(op->ctx, s);
   delete op;
   return BRTM_OK;
 }
@@ -314,6 +331,7 @@ int brtm_op_prepare(brtm_op *op) {
     OP_TRY(load_shot(op, s));
     OP_TRY(brtm_source_prop(op->ctx, op->d.jt, op->nt_fd, op->d.jimg, op->d.store, op->d.ckpt,
                             nullptr, nullptr));
+    op->ctx->stores[s].owner = op;
   }
   op->prepared = true;
   return BRTM_OK;
@@ -345,6 +363,7 @@ int brtm_op_forward(brtm_op *op, int add, const brtm_vec *model, brtm_vec *data)
   const float inv_dt = 1.f / d.dt_data;
   const long nrec = op->nrec();
   for (int s = 0; s < d.nshots; ++s) {
+    OP_TRY(check_store(op, s));
     OP_TRY(load_shot(op, s));
     OP_TRY(born_forward(c, d.jt, op->nt_fd, d.jimg, false));
     BRTM_HIP(launch_trace_to_data(c->traces.as<float>(),
@@ -369,6 +388,7 @@ int brtm_op_adjoint(brtm_op *op, int add, brtm_vec *model, const brtm_vec *data)
   const float inv_dt = 1.f / d.dt_data;
   const long nrec = op->nrec();
   for (int s = 0; s < d.nshots; ++s) {
+    OP_TRY(check_store(op, s));
     OP_TRY(load_shot(op, s));
     BRTM_HIP(launch_data_to_trace(data->p() + (size_t)s * nrec * d.nt_data, c->traces.as<float>(),
                                   (int)nrec, op->ntr_d, d.nt_data, inv_dt, st));
@@ -406,6 +426,7 @@ int brtm_op_illumination(brtm_op *op, brtm_vec *out) {
   float *acc = op->work.as<float>();
   BRTM_HIP(hipMemsetAsync(acc, 0, sizeof(float) * (size_t)c->ncell, st));
   for (int s = 0; s < d.nshots; ++s) {
+    OP_TRY(check_store(op, s));
     OP_TRY(brtm_select_shot(c, s));
     OP_TRY(illumination(c, acc));
   }

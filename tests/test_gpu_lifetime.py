@@ -72,3 +72,40 @@ def test_python_objects_outlive_their_device():
     z = DeviceVector.from_array(d2, np.ones(10, np.float32))
     d2.close()                  # closes z first
     assert z.h is None
+
+
+def test_operators_sharing_a_context_keep_their_own_stores(dev):
+    """Operators on one context share its per-shot source-wavefield stores.
+    Destroying an older operator (explicitly, or late, by garbage
+    collection) leaves the stores a newer one prepared, and an operator
+    whose stores another one has replaced refuses to run until it prepares
+    them again, instead of imaging with the other operator's fields."""
+    print("lifetime: operators sharing a context", file=sys.stderr, flush=True)
+    from sep_code_amd import configs
+    from sep_code_amd.operator import RtmZeroOp
+    from sep_code_amd._lib import BrtmError
+    w = configs.lsrtm(seed=5, n_side=32, nt=40, nshots=(2, 1), rec_step=3, store="f32")
+    dev.create_space(w.d, w.bc_a, w.bc_b, w.n)
+    dev.set_velocity(w.V)
+    dev.set_imaging_scale(w.cimg)
+    n1, n2, n3 = w.n
+    nb = w.nbound
+    mbox = (n1 - 2 * nb, n2 - 2 * nb, n3 - 2 * nb, nb, nb, nb)
+    old = RtmZeroOp(dev, w, store="f32", model_box=mbox)
+    old.prepare()
+    new = RtmZeroOp(dev, w, store="f32", model_box=mbox)
+    new.prepare()
+    m = new.model_vector(np.random.default_rng(5).standard_normal(new.model_size))
+    d1 = new.data_vector()
+    new.forward(False, m, d1)
+    ref = d1.numpy()
+    assert np.abs(ref).max() > 0
+    # the older operator's shots now hold the newer one's fields
+    with pytest.raises(BrtmError) as e:
+        old.forward(False, m, old.data_vector())
+    assert e.value.code == BRTM_ESTATE and "another operator" in str(e.value)
+    old.close()  # must not release the newer operator's stores
+    d2 = new.data_vector()
+    new.forward(False, m, d2)
+    np.testing.assert_array_equal(d2.numpy(), ref)
+    new.close()
