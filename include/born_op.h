@@ -83,9 +83,14 @@ typedef struct {
 /* Replaces rtm_zero_op::rtm_zero_op (Born/rtm_zero_op_3d.cpp:8-37): copies
  * the geometry; the context must already hold the space and velocity. */
 int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out);
+/* Releases the per-shot stores this operator's brtm_op_prepare made (not
+ * those another operator on the context has prepared since). */
 int brtm_op_destroy(brtm_op *op);
 /* Replaces create_source_fields (Born/rtm_zero_op_3d.cpp:38-85): propagate
- * and store every shot's source wavefield. */
+ * and store every shot's source wavefield.  The stores are the context's,
+ * one per shot index: after another operator on the same context prepares
+ * a shot, this operator's forward, adjoint and illumination return
+ * BRTM_ESTATE until it prepares again. */
 int brtm_op_prepare(brtm_op *op);
 long brtm_op_model_size(const brtm_op *op);
 long brtm_op_data_size(const brtm_op *op);
