@@ -247,12 +247,13 @@ LSRTM_WORKLOAD = ("lsrtm256: LSRTM, 256x256x256 FP32, 8 shots, nt=2500, FP16 sou
                   "snapshots every 4 steps, Born data + 1 % Gaussian noise")
 
 
-def _lsrtm_setup(log):
+def _lsrtm_setup(log, graphs=-1):
     from sep_code_amd import configs, lsrtm
     from sep_code_amd.device import BornDevice
     t0 = time.perf_counter()
     w = configs.lsrtm()
     dev = BornDevice(0)
+    dev.set_graphs(graphs)
     op, tprep = lsrtm.setup_operator(dev, w)
     d, floor, e = lsrtm.observed_data(op, w, keep_noise=True)
     log(f"lsrtm256 set up in {time.perf_counter() - t0:.1f}s (source fields {tprep:.1f}s), "
@@ -317,7 +318,7 @@ def lsrtm_main(a):
     adjoint's dominant kernel (26 algorithmic bytes per cell)."""
     from sep_code_amd.lsrtm_bench import DEFAULT_WEIGHTS, compare
     a.lsrtm_weights = a.lsrtm_weights or DEFAULT_WEIGHTS
-    w, dev, op, d, floor, e = _lsrtm_setup(log)
+    w, dev, op, d, floor, e = _lsrtm_setup(log, a.graphs)
     t0 = time.perf_counter()
     solvers = set(a.lsrtm_solvers.split(","))
     out = compare(dev, w, op, d, e, floor, niter=a.lsrtm_niter, tol=1e-2,
@@ -350,7 +351,8 @@ def lsrtm_main(a):
         "steps": 1, "warmup": 0, "ms_per_step": round(el * 1e3, 1), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": LSRTM_WORKLOAD, "grid": list(w.n), "nt": w.nt,
-                   "shots": len(w.shots), "store": w.store, "parallelism": "single GPU"},
+                   "shots": len(w.shots), "store": w.store, "graphs": a.graphs,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
