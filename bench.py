@@ -299,9 +299,9 @@ def lsrtm_slice(iters, log):
     tw = time.perf_counter() - t0
     pre = net.precond(lsrtm.model_box(w)[:3], overlap=8, split_weight=W)
     r, m = lsrtm.run_lsrtm(dev, w, op, d, niter=iters, rel_tol=0.0, name="fpcg", noise=e,
-                           precond=pre, history=4)
+                           precond=pre, history=8)
     m.close()
-    res["fpcg"] = rec(r, {"illumination_s": round(tw, 3), "history": 4, "guard_mode": 1,
+    res["fpcg"] = rec(r, {"illumination_s": round(tw, 3), "history": 8, "guard_mode": 1,
                           "guard_diagonal_steps": net.guard_count()})
     log(f"lsrtm learned FCG: {iters} it, {r.seconds / iters:.3f} s/it, |Lm-d|/|d| "
         f"{r.data_rel_resid_recomputed:.4f}, noise-free misfit {r.rel_resid_recomputed:.4f}")

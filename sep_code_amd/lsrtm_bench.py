@@ -48,7 +48,7 @@ def _entry(r, extra=None):
 
 
 def compare(dev, w, op, d, e, floor, *, niter=400, tol=1e-2, weights=DEFAULT_WEIGHTS, batch=8,
-            even=False, guard="0.8,0.05", history=4, diag_eps="", split_eps=0.01,
+            even=False, guard="0.8,0.05", history=8, diag_eps="", split_eps=0.01,
             skip_plain=False, quiet=True, log=None, guard_mode=1):
     """Plain CG, optionally illumination-diagonal PCG for each eps in the
     comma list diag_eps, and learned flexible CG with the network in
@@ -131,8 +131,9 @@ def main():
                     help="repeat_cos,min_cos of the split preconditioner's guard")
     ap.add_argument("--quiet", action="store_true",
                     help="no per-iteration progress lines on stderr")
-    ap.add_argument("--history", type=int, default=4,
-                    help="steps kept by the flexible CG of the learned run")
+    ap.add_argument("--history", type=int, default=8,
+                    help="steps kept by the flexible CG of the learned run (each holds "
+                         "one data-space vector: 5.3 GB at 256^3)")
     ap.add_argument("--diag-eps", default="",
                     help="comma list: also run CG preconditioned by the illumination "
                          "diagonal (I + eps max I)^-1 for each eps")

@@ -22,7 +22,8 @@ holds the whole survey.  Additional: store=f32|bf16|f16|ckpt (source-wavefield
 store; ckpt for fwd and rtm, bf16 for inv by default), ckpt=, jimg= (imaging
 cadence, default jt),
 rel_tol= (inv: stop at that relative residual), precond=none|diag|learned
-(inv; illum_eps= the illumination floor), history= (inv, flexible-CG steps),
+(inv; illum_eps= the illumination floor), history= (inv, flexible-CG steps,
+8 by default with precond=learned; each holds one data-space vector),
 eps= (inv: reg_solver with eps times the identity), tmute= vmute= (inv: the
 tmute data weight, mute before tmute + x/vmute), device= (GPU index).
 
@@ -206,7 +207,7 @@ def main(argv=None):
                         "precond_split_e0.01.npy")))
                     W = lsrtm.illumination_weight(op, p.get_float("illum_eps", 0.01), power=0.5)
                     precond = net.precond(mbox[:3], overlap=8, split_weight=W)
-                    history = history or 4
+                    history = history or 8
             wt = None
             if p.has("tmute"):
                 # the data weight of the reference's commented-out tmute

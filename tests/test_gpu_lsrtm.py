@@ -42,7 +42,7 @@ def test_trained_network_beats_plain_and_diagonal(dev):
     W = lsrtm.illumination_weight(op, 0.01, power=0.5)
     pre = net.precond(lsrtm.model_box(w)[:3], overlap=8, split_weight=W)
     learned, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=niter, rel_tol=0.0, noise=e,
-                                 precond=pre, history=4)
+                                 precond=pre, history=8)
     r = {k: v.rel_resid_recomputed for k, v in
          (("plain", plain), ("diag", diag), ("learned", learned))}
     announce(f"  misfit after {niter} iterations: {r}; guard took the diagonal on "
@@ -80,7 +80,7 @@ def test_trained_network_reaches_the_tolerance_sooner(dev):
     W = lsrtm.illumination_weight(op, 0.01, power=0.5)
     pre = net.precond(lsrtm.model_box(w)[:3], overlap=8, split_weight=W)
     learned, _ = lsrtm.run_lsrtm(dev, w, op, d, niter=250, rel_tol=1e-2, noise=e,
-                                 precond=pre, history=4)
+                                 precond=pre, history=8)
     announce(f"  diagonal {diag.iterations} it {diag.seconds:.1f} s; learned "
              f"{learned.iterations} it {learned.seconds:.1f} s, guard "
              f"{net.guard_count()}")
