@@ -355,6 +355,66 @@ __global__ void k_first_fwd(const float *__restrict__ x, const float *__restrict
   }
 }
 
+// Four consecutive x positions per thread (Wd % 4 == 0, 16-byte aligned x and
+// y, cout % 8 == 0): each of the nine (z, y) rows of the 3x3x3 neighbourhood
+// is one 16-byte load plus its two end cells, a quarter of the loads of one
+// position per thread.  The taps and their order are k_first_fwd's: the same
+// arithmetic for every output.
+__global__ void k_first_fwd4(const float *__restrict__ x, const float *__restrict__ W,
+                             const float *__restrict__ b, uint16_t *y, int ycs, int cout, int N,
+                             int D, int H, int Wd) {
+  const int wq = Wd >> 2;
+  const long nq = (long)N * D * H * wq;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (long)gridDim.x * blockDim.x) {
+    const int x0 = (int)(q % wq) * 4;
+    long r = q / wq;
+    const int yv = (int)(r % H);
+    r /= H;
+    const int zv = (int)(r % D);
+    const long nn = r / D;
+    // nb[dz * 3 + dy][k]: the input at (zv + dz - 1, yv + dy - 1, x0 + k - 1)
+    float nb[9][6];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int z2 = zv + t / 3 - 1, y2 = yv + t % 3 - 1;
+      float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+      float lo = 0.f, hi = 0.f;
+      if (z2 >= 0 && z2 < D && y2 >= 0 && y2 < H) {
+        const float *row = x + ((nn * D + z2) * H + y2) * Wd;
+        c = *reinterpret_cast<const float4 *>(row + x0);
+        if (x0 > 0) lo = row[x0 - 1];
+        if (x0 + 4 < Wd) hi = row[x0 + 4];
+      }
+      nb[t][0] = lo;
+      nb[t][1] = c.x;
+      nb[t][2] = c.y;
+      nb[t][3] = c.z;
+      nb[t][4] = c.w;
+      nb[t][5] = hi;
+    }
+    const long p0 = 4 * q;  // the position of x0
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      for (int c0 = 0; c0 < cout; c0 += 8) {
+        uint32_t w[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          float s0 = b[c0 + 2 * h], s1 = b[c0 + 2 * h + 1];
+#pragma unroll
+          for (int tap = 0; tap < 27; ++tap) {
+            const float v = nb[tap / 3][j + tap % 3];
+            s0 = fmaf(W[tap * cout + c0 + 2 * h], v, s0);
+            s1 = fmaf(W[tap * cout + c0 + 2 * h + 1], v, s1);
+          }
+          w[h] = (uint32_t)f2bf(fmaxf(s0, 0.f)) | ((uint32_t)f2bf(fmaxf(s1, 0.f)) << 16);
+        }
+        *reinterpret_cast<uint4 *>(y + (p0 + j) * ycs + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    }
+  }
+}
+
 __global__ void k_first_wgrad(const float *__restrict__ x, const uint16_t *__restrict__ dy,
                               int dycs, int cout, float *dW, float *db, int N, int D, int H,
                               int Wd) {
@@ -685,8 +745,15 @@ hipError_t unpack_grad(const float *gp, float *g, int cin, int cout, hipStream_t
 
 hipError_t conv3d_first_fwd(const float *x, const float *W, const float *b, Act y, Vol v,
                             hipStream_t st) {
-  k_first_fwd<<<grid1((long)v.n * v.spatial()), 256, 0, st>>>(x, W, b, (uint16_t *)y.p, y.cs,
-                                                               y.c, v.n, v.d, v.h, v.w);
+  const bool quad = v.w % 4 == 0 && y.c % 8 == 0 && y.cs % 8 == 0 &&
+                    reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                    reinterpret_cast<uintptr_t>(y.p) % 16 == 0;
+  if (quad)
+    k_first_fwd4<<<grid1((long)v.n * v.spatial() / 4), 256, 0, st>>>(
+        x, W, b, (uint16_t *)y.p, y.cs, y.c, v.n, v.d, v.h, v.w);
+  else
+    k_first_fwd<<<grid1((long)v.n * v.spatial()), 256, 0, st>>>(x, W, b, (uint16_t *)y.p, y.cs,
+                                                                 y.c, v.n, v.d, v.h, v.w);
   return hipGetLastError();
 }
 
