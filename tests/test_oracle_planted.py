@@ -1,8 +1,10 @@
-"""The closed-form time-recursion tests (tests/test_oracle.py) catch the
-plausible indexing mistakes: each is planted in a copy of the oracle's C
-source (oracle/born_oracle.c), built into a separate FP64 library, and the
-closed-form tests run against it must fail, while they pass on the oracle
-itself.  CPU only; the copies are built in a temporary directory."""
+"""The oracle's pins (tests/test_oracle.py) catch the plausible mistakes:
+each is planted in a copy of the oracle's C source (oracle/born_oracle.c),
+built into a separate FP64 library, and the pinning tests run against it
+must fail, while they pass on the oracle itself.  Two groups: the time
+indexing (closed-form recursions) and the stencil and sponge (Fornberg
+weights, polynomial and plane-wave exactness, the sponge formulas, the VL/LV
+transpose).  CPU only; the copies are built in a temporary directory."""
 import os
 import shutil
 import subprocess
@@ -48,6 +50,30 @@ MUTANTS = [
      "                   (real)1, *la);"),
 ]
 
+# after Born/wave_fkernel.3d8o.cu:31-106 (the VL update and its taps),
+# :183-210 (the sponge: w on the update, w^2 on p_old; the edge distance)
+STENCIL_MUTANTS = [
+    ("y taps with the x spacing",
+     "    c[2 + 3 * k] = (real)(C2 * f[k]);",
+     "    c[2 + 3 * k] = (real)(C1 * f[k]);"),
+    ("fourth-distance weight with the wrong sign",
+     "  static const double f[4] = {1.0, -1.0 / 8.0, 1.0 / 63.0, -1.0 / 896.0};",
+     "  static const double f[4] = {1.0, -1.0 / 8.0, 1.0 / 63.0, 1.0 / 896.0};"),
+    ("p_old damped by w instead of w^2",
+     "        pn[i] = w * (P->V[i] * l + 2 * p[i]) - w * w * po[i];",
+     "        pn[i] = w * (P->V[i] * l + 2 * p[i]) - w * po[i];"),
+    ("symmetric edge distance",
+     "  int e = imin(i1 - RAD, n1 - RAD - i1);",
+     "  int e = imin(i1 - RAD, n1 - RAD - 1 - i1);"),
+]
+
+STENCIL_PINS = [(name, [()]) for name in (
+    "test_coeffs_are_fornberg_weights", "test_1d_weights_exact_on_polynomials",
+    "test_stencil_laplacian_of_polynomial_anisotropic", "test_stencil_symbol_on_plane_wave",
+    "test_sponge_table_closed_form", "test_sponge_edge_distance_matches_reference_formula",
+    "test_lv_step_is_transpose_of_vl_step", "test_undamped_leapfrog_is_time_reversible",
+    "test_sponge_dissipates_energy")]
+
 CLOSED_FORM = [
     ("test_source_prop_one_sample_delay_jt1", [()]),
     ("test_source_prop_closed_form_two_steps_per_sample", [(2, 2), (2, 3)]),
@@ -56,9 +82,9 @@ CLOSED_FORM = [
 ]
 
 
-def _failing():
+def _failing(tests):
     out = []
-    for name, cases in CLOSED_FORM:
+    for name, cases in tests:
         for args in cases:
             try:
                 getattr(T, name)(*args)
@@ -87,15 +113,15 @@ def _use(lib_dir):
         O.HERE = here
 
 
-def test_closed_form_tests_pass_on_the_oracle():
-    assert _failing() == []
+def test_pinning_tests_pass_on_the_oracle():
+    assert _failing(CLOSED_FORM) == []
+    assert _failing(STENCIL_PINS) == []
 
 
-@pytest.mark.parametrize("label,orig,planted", MUTANTS, ids=[m[0] for m in MUTANTS])
-def test_closed_form_tests_catch_a_planted_mistake(builds, label, orig, planted):
+def _planted(builds, tag, label, orig, planted, tests):
     src = open(SRC_PATH).read()
     assert src.count(orig) == 1, f"the oracle text for '{label}' has changed"
-    d = os.path.join(builds, str(MUTANTS.index((label, orig, planted))))
+    d = os.path.join(builds, tag)
     os.makedirs(d)
     c_path = os.path.join(d, "born_oracle.c")
     with open(c_path, "w") as f:
@@ -106,9 +132,23 @@ def test_closed_form_tests_catch_a_planted_mistake(builds, label, orig, planted)
     saved = O._LIBS.get(64)
     try:
         _use(d)
-        failing = _failing()
+        return _failing(tests)
     finally:
         O._LIBS.pop(64, None)
         if saved is not None:
             O._LIBS[64] = saved
+
+
+@pytest.mark.parametrize("label,orig,planted", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_closed_form_tests_catch_a_planted_mistake(builds, label, orig, planted):
+    tag = f"t{[m[0] for m in MUTANTS].index(label)}"
+    failing = _planted(builds, tag, label, orig, planted, CLOSED_FORM)
     assert failing, f"no closed-form test caught: {label}"
+
+
+@pytest.mark.parametrize("label,orig,planted", STENCIL_MUTANTS,
+                         ids=[m[0] for m in STENCIL_MUTANTS])
+def test_stencil_pins_catch_a_planted_mistake(builds, label, orig, planted):
+    tag = f"s{[m[0] for m in STENCIL_MUTANTS].index(label)}"
+    failing = _planted(builds, tag, label, orig, planted, STENCIL_PINS)
+    assert failing, f"no stencil or sponge pin caught: {label}"
