@@ -140,7 +140,7 @@ typedef struct {
   double *data_resid_hist;
 } brtm_cg_params;
 
-#define BRTM_CG_MAX_HISTORY 8
+#define BRTM_CG_MAX_HISTORY 16
 
 typedef struct {
   int iterations;           /* steps taken                                      */

@@ -133,7 +133,7 @@ def main():
                     help="no per-iteration progress lines on stderr")
     ap.add_argument("--history", type=int, default=8,
                     help="steps kept by the flexible CG of the learned run (each holds "
-                         "one data-space vector: 5.3 GB at 256^3)")
+                         "one data-space vector: 1.31 GB at 256^3)")
     ap.add_argument("--diag-eps", default="",
                     help="comma list: also run CG preconditioned by the illumination "
                          "diagonal (I + eps max I)^-1 for each eps")
