@@ -671,6 +671,57 @@ def test_receiver_index_hand_example():
     assert O.receiver_index(10, 12, [2.7], [3.1], [5.9])[0] == 2 + 3 * 10 + (5 + 4) * 120
 
 
+def test_inactive_receivers_hand_example():
+    """A receiver outside the update set [4, n-4) on any axis is marked -1.
+    The first and last updated cells of each axis are kept and their outer
+    neighbours dropped; the axes have different lengths, so a swapped axis
+    or a wrong stride fails."""
+    n = (11, 13, 15)  # update sets [4, 7), [4, 9), [4, 11)
+    centre = (5, 6, 7)
+
+    def loc(i):
+        return i[0] + n[0] * (i[1] + n[1] * i[2])
+
+    keep = {centre: True, (4, 4, 4): True, (6, 8, 10): True}
+    for ax in range(3):
+        for i, kept in ((3, False), (4, True), (n[ax] - 5, True), (n[ax] - 4, False)):
+            c = list(centre)
+            c[ax] = i
+            keep[tuple(c)] = kept
+    locs = [loc(k) for k in keep]
+    n1, n2, n3 = n
+    got = O.mask_inactive(locs, (n1, n2, n3))
+    assert got.dtype == np.int32
+    for (k, kept), l, g in zip(keep.items(), locs, got):
+        assert g == (l if kept else -1), k
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 100003])
+def test_dot_accumulates_products_in_fp64(n):
+    """orc_dot (Born/hypercube_float.cpp:123-132) adds the products in FP64 in
+    both builds. A product of two FP32 values is exact in FP64, so the FP32
+    build returns the FP64 build's bits on the same values, and both are
+    within n ulp-sized steps of the exact sum; a product or a sum rounded to
+    FP32 would miss both."""
+    r = rng(31)
+    a = r.standard_normal(n).astype(np.float32)
+    b = r.standard_normal(n).astype(np.float32)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    d32 = O.lib(32).orc_dot(n, a.ctypes.data, b.ctypes.data)
+    d64 = O.lib(64).orc_dot(n, a64.ctypes.data, b64.ctypes.data)
+    prods = a64 * b64
+    exact = math.fsum(prods.tolist())
+    assert d32 == d64
+    assert abs(d64 - exact) <= n * 2.0 ** -53 * float(np.abs(prods).sum())
+    if n > 1:
+        # the same sum accumulated in FP32 is far outside that bound
+        s = np.float32(0)
+        for p in prods[:1000].astype(np.float32):
+            s = np.float32(s + p)
+        part = math.fsum(prods[:1000].tolist())
+        assert abs(float(s) - part) > min(n, 1000) * 2.0 ** -53 * float(np.abs(prods[:1000]).sum())
+
+
 # --- FP32 over long runs: the error does not accumulate, and the summed ----
 # --- (Henrici) form of the update does not reduce it ------------------------
 def _fp32_runs(n=36, nt=1000, jimg=100, bc_a=8.0, bc_b=0.02):
