@@ -667,6 +667,13 @@ def test_source_geometry_hand_example():
     assert np.all(tr[:, :4] == 0)
 
 
+def test_wavelet_derivative_hand_example():
+    """Born/source_func_3d.cpp:9-13: w'[t] = (w[t+1] - w[t]) / dt, and the
+    last sample 0 (a forward difference: the first output sees w[1])."""
+    np.testing.assert_array_equal(O.wavelet_derivative([1.0, 3.0, 6.0, 10.0], 0.5),
+                                  [4.0, 6.0, 8.0, 0.0])
+
+
 def test_receiver_index_hand_example():
     assert O.receiver_index(10, 12, [2.7], [3.1], [5.9])[0] == 2 + 3 * 10 + (5 + 4) * 120
 

@@ -178,3 +178,25 @@ def test_problem_from_sep_geometry(tmp_path):
 def test_cli_usage_without_program():
     assert rtm_cli.main([]) == 2
     assert rtm_cli.main(["migrate"]) == 2
+
+
+def test_geometry_matches_the_oracle():
+    """The library's source points, source traces, receiver cells and wavelet
+    derivative against the oracle's restatement (Born/source_func_3d.cpp:9-13,
+    100-133; Born/map_data_3d.cpp:17-29) at random fractional positions."""
+    from oracle import born_oracle as O
+    r = np.random.default_rng(8)
+    n1, n2, ntr = 40, 36, 58
+    for _ in range(25):
+        fx, fy, fz = r.uniform(1.0, 38.0), r.uniform(1.0, 34.0), r.uniform(1.0, 20.0)
+        w = r.standard_normal(int(r.integers(10, 60)))
+        wd = G.wavelet_derivative(w, 0.002)
+        np.testing.assert_array_equal(wd, O.wavelet_derivative(w, 0.002))
+        wd = wd[:ntr - 4]
+        locs, tr = G.source_points(n1, n2, fx, fy, fz, wd, ntr)
+        olocs, otr = O.source_geometry(n1, n2, fz, fx, fy, wd, ntr)
+        np.testing.assert_array_equal(locs, olocs)
+        np.testing.assert_allclose(tr, otr, rtol=1e-6, atol=1e-30)
+    sx, sy, sz = r.uniform(0, 39, 200), r.uniform(0, 35, 200), r.uniform(0, 20, 200)
+    np.testing.assert_array_equal(G.receiver_locs(n1, n2, sx, sy, sz),
+                                  O.receiver_index(n1, n2, sx, sy, sz))
