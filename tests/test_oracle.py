@@ -465,6 +465,17 @@ def test_dz_and_dt_closed_forms():
     assert np.allclose(dt[1:], -1.0) and dt[0] == 0.0
 
 
+def test_window_hand_example():
+    """Born/map_data_3d.cpp:44-76: data sample t of each trace is sample t+4
+    of that trace's ntb-long device trace; the adjoint puts it back there
+    and leaves the first 4 and the last ntb-4-nt samples zero."""
+    m = np.arange(20.0).reshape(2, 10)  # two traces, ntb = 10
+    np.testing.assert_array_equal(O.window_forward(m, 5), [[4, 5, 6, 7, 8],
+                                                           [14, 15, 16, 17, 18]])
+    np.testing.assert_array_equal(O.window_adjoint([[1.0, 2.0], [3.0, 4.0]], 7),
+                                  [[0, 0, 0, 0, 1, 2, 0], [0, 0, 0, 0, 3, 4, 0]])
+
+
 # --- A14 interpolation tables ----------------------------------------------
 def test_levinson_matches_scipy_toeplitz():
     r = rng(2)

@@ -4,12 +4,16 @@ built into a separate FP64 library, and the pinning tests run against it
 must fail, while they pass on the oracle itself.  Two groups: the time
 indexing (closed-form recursions) and the stencil and sponge (Fornberg
 weights, polynomial and plane-wave exactness, the sponge formulas, the VL/LV
-transpose).  CPU only; the copies are built in a temporary directory."""
+transpose).  CPU only; the copies are built in a temporary directory.  A
+third group swaps mistaken versions of the oracle's numpy operators (D_t,
+D_z, the data window, the wavelet derivative, the receiver mask) into the
+module and runs their closed forms and hand examples."""
 import os
 import shutil
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 import oracle.born_oracle as O
@@ -152,3 +156,118 @@ def test_stencil_pins_catch_a_planted_mistake(builds, label, orig, planted):
     tag = f"s{[m[0] for m in STENCIL_MUTANTS].index(label)}"
     failing = _planted(builds, tag, label, orig, planted, STENCIL_PINS)
     assert failing, f"no stencil or sponge pin caught: {label}"
+
+
+# Mistakes in the oracle's numpy operators, each made consistently in the
+# forward and the adjoint (so the adjoint-pair tests still pass): after
+# Born/deriv_3d.cpp:4-46, Born/laplac_3d.cpp:4-65, Born/map_data_3d.cpp:44-76,
+# Born/source_func_3d.cpp:9-13 and the update set [4, n-4).
+def _dt_fwd_diff(m, dt):
+    m = np.asarray(m, np.float64)
+    d = np.zeros_like(m)
+    d[..., :-1] = (m[..., :-1] - m[..., 1:]) / dt
+    return d
+
+
+def _dt_fwd_diff_adj(d, dt):
+    d = np.asarray(d, np.float64)
+    m = np.zeros_like(d)
+    m[..., :-1] += d[..., :-1] / dt
+    m[..., 1:] -= d[..., :-1] / dt
+    return m
+
+
+def _dz_down(m):
+    m = np.asarray(m, np.float64)
+    d = np.zeros_like(m)
+    d[:-1] = m[:-1] - m[1:]
+    return d
+
+
+def _dz_down_adj(d):
+    d = np.asarray(d, np.float64)
+    m = np.zeros_like(d)
+    m[:-1] += d[:-1]
+    m[1:] -= d[:-1]
+    return m
+
+
+def _window3(m, nt):
+    return np.array(np.asarray(m)[..., 3:3 + nt], np.float64)
+
+
+def _window3_adj(d, ntb):
+    d = np.asarray(d, np.float64)
+    m = np.zeros(d.shape[:-1] + (ntb,), np.float64)
+    m[..., 3:3 + d.shape[-1]] = d
+    return m
+
+
+def _wavelet_backward(w, dt):
+    w = np.asarray(w, np.float64)
+    out = np.zeros_like(w)
+    out[1:] = (w[1:] - w[:-1]) / dt
+    return out
+
+
+def _mask(lo, hi):
+    def f(locs, n):
+        n1, n2, n3 = n
+        rl = np.asarray(locs, np.int64).copy()
+        i1, i2, i3 = rl % n1, (rl // n1) % n2, rl // (n1 * n2)
+        ok = ((i1 >= lo) & (i1 < n1 - hi) & (i2 >= lo) & (i2 < n2 - hi) & (i3 >= lo)
+              & (i3 < n3 - hi))
+        rl[~ok] = -1
+        return rl.astype(np.int32)
+    return f
+
+
+def _mask_xy_swapped(locs, n):
+    n1, n2, n3 = n
+    rl = np.asarray(locs, np.int64).copy()
+    i1, i2, i3 = rl % n2, (rl // n2) % n1, rl // (n1 * n2)
+    ok = (i1 >= 4) & (i1 < n1 - 4) & (i2 >= 4) & (i2 < n2 - 4) & (i3 >= 4) & (i3 < n3 - 4)
+    rl[~ok] = -1
+    return rl.astype(np.int32)
+
+
+PY_MUTANTS = [
+    ("D_t with the opposite sign",
+     {"dt_forward": lambda m, dt: -_ORIG["dt_forward"](m, dt),
+      "dt_adjoint": lambda d, dt: -_ORIG["dt_adjoint"](d, dt)}),
+    ("D_t as the difference toward the next sample",
+     {"dt_forward": _dt_fwd_diff, "dt_adjoint": _dt_fwd_diff_adj}),
+    ("D_z as the difference toward the next slice",
+     {"dz_forward": _dz_down, "dz_adjoint": _dz_down_adj}),
+    ("data window dropping 3 samples", {"window_forward": _window3,
+                                        "window_adjoint": _window3_adj}),
+    ("wavelet derivative as a backward difference", {"wavelet_derivative": _wavelet_backward}),
+    ("receivers on the last halo cell kept", {"mask_inactive": _mask(4, 3)}),
+    ("receivers on the first updated cell dropped", {"mask_inactive": _mask(5, 4)}),
+    ("receiver mask with x and y swapped", {"mask_inactive": _mask_xy_swapped}),
+]
+_ORIG = {name: getattr(O, name) for m in PY_MUTANTS for name in m[1]}
+
+PY_PINS = [(name, [()]) for name in (
+    "test_dz_dt_window_are_adjoint_pairs", "test_dz_and_dt_closed_forms",
+    "test_window_hand_example", "test_wavelet_derivative_hand_example",
+    "test_inactive_receivers_hand_example")]
+
+
+def test_numpy_pins_pass_on_the_oracle():
+    assert _failing(PY_PINS) == []
+
+
+@pytest.mark.parametrize("label,patch", PY_MUTANTS, ids=[m[0] for m in PY_MUTANTS])
+def test_numpy_pins_catch_a_planted_mistake(label, patch):
+    try:
+        for name, fn in patch.items():
+            setattr(O, name, fn)
+        failing = _failing(PY_PINS)
+    finally:
+        for name in patch:
+            setattr(O, name, _ORIG[name])
+    assert failing, f"no pin caught: {label}"
+    # the adjoint pair still holds for every mutated pair, so it is the
+    # closed forms and hand examples that catch them
+    assert "test_dz_dt_window_are_adjoint_pairs()" not in failing
