@@ -96,7 +96,7 @@ int brtm_solver_create(int kind, brtm_linop *op, const brtm_vec *data, brtm_lino
 int brtm_solver_solve(brtm_solver *s, const brtm_cg_params *params, double *resid_hist,
                       double *time_hist, brtm_cg_stats *stats);
 /* lin_solver::return_model (m) / prec_solver::return_model (P m) into
- * model (Born/lin_solver.cpp:144-160). */
+ * model (Born/lin_solver.cpp:144-155). */
 int brtm_solver_return_model(brtm_solver *s, brtm_vec *model);
 /* update_model: model += m (or P m) (Born/lin_solver.cpp:145,152-154). */
 int brtm_solver_update_model(brtm_solver *s, brtm_vec *model);
