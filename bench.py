@@ -314,7 +314,7 @@ def lsrtm_slice(iters, log):
 def lsrtm_main(a):
     """--workload lsrtm256: plain CG, the illumination diagonal and learned
     flexible CG to the tolerance; the line's value is the learned run's
-    time to it.  The roofline object covers the BF16 imaging step, the
+    time to it.  The roofline object covers the FP16 imaging step, the
     adjoint's dominant kernel (26 algorithmic bytes per cell)."""
     from sep_code_amd.lsrtm_bench import DEFAULT_WEIGHTS, compare
     a.lsrtm_weights = a.lsrtm_weights or DEFAULT_WEIGHTS
