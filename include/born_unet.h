@@ -66,12 +66,14 @@ int brtm_unet_set_grid(brtm_unet *net, int m1, int m2, int m3, int overlap);
  * brtm_unet_set_guard).  The vector is not copied: keep it alive while it is
  * set.  Setting it (or the grid) starts a new solve for the guard. */
 int brtm_unet_set_split_weight(brtm_unet *net, const brtm_vec *w);
-/* Guard of the split preconditioner: the diagonal direction W^2 g replaces
- * the network's z when cos(z, g) < min_cos (z brings nothing the gradient
- * does not) or when cos(z, z_prev) > repeat_cos, z_prev being the network's
- * previous output (the network has settled on a fixed direction, which
- * stalls flexible CG).  Defaults 0.05 and 0.8.  brtm_unet_guard_count
- * returns how many calls took W^2 g since the split weight was set. */
+/* Thresholds of the split preconditioner's guard (defaults 0.8 and 0.05).
+ * min_cos: a z with cos(z, g) < min_cos brings nothing the gradient does
+ * not, and becomes W^2 g.  repeat_cos: a network output (mode 0) or
+ * correction (mode 1, the default) whose cosine with the previous one
+ * exceeds it has settled on a fixed direction, which stalls flexible CG;
+ * brtm_unet_set_guard_mode says what each mode then does.
+ * brtm_unet_guard_count returns how many calls the guard acted on since the
+ * split weight was set. */
 int brtm_unet_set_guard(brtm_unet *net, double repeat_cos, double min_cos);
 int brtm_unet_guard_count(const brtm_unet *net);
 /* Guard mode 1 (the default): z = W^2 g + c, c = the network's correction
