@@ -635,6 +635,12 @@ def test_lin_solve_forms_reach_their_closed_forms():
     WLP = WL @ P
     p = np.linalg.solve(WLP.T @ WLP + eps ** 2 * np.eye(10), WLP.T @ (W @ d))
     np.testing.assert_allclose(m, P @ p, rtol=1e-8, atol=1e-10)
+    # a banded weight is not its own transpose (a diagonal one is): applied
+    # transposed it gives another weighted solution
+    Wb = W + 0.4 * np.eye(W.shape[0], k=1)
+    m, _ = O.lin_solve("simple", L, d, 30, W=Wb, m0=m0)
+    np.testing.assert_allclose(m, np.linalg.lstsq(Wb @ L, Wb @ r, rcond=None)[0], rtol=1e-8,
+                               atol=1e-10)
     # eps = 0 and identity weight reduce to plain cgstep
     m1, res1 = O.lin_solve("reg", L, d, 6, aux=R, eps=0.0)
     m2, res2 = O.cgstep_solve(L, d, 6)

@@ -7,7 +7,8 @@ weights, polynomial and plane-wave exactness, the sponge formulas, the VL/LV
 transpose).  CPU only; the copies are built in a temporary directory.  A
 third group swaps mistaken versions of the oracle's numpy operators (D_t,
 D_z, the data window, the wavelet derivative, the receiver mask) into the
-module and runs their closed forms and hand examples."""
+module and runs their closed forms and hand examples; a fourth wraps the
+solver forms (lin_solve) with assembly mistakes and runs their closed forms."""
 import os
 import shutil
 import subprocess
@@ -271,3 +272,76 @@ def test_numpy_pins_catch_a_planted_mistake(label, patch):
     # the adjoint pair still holds for every mutated pair, so it is the
     # closed forms and hand examples that catch them
     assert "test_dz_dt_window_are_adjoint_pairs()" not in failing
+
+
+# Mistakes in how the three solver forms are assembled (Born/lin_solver.cpp:
+# 7-53, 79-101), planted around the oracle's lin_solve by changing what it is
+# given or what it returns.
+_LIN = O.lin_solve
+
+
+def _lin(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    return _LIN(kind, L, d, niter, aux=aux, eps=eps, W=W, m0=m0, history=history)
+
+
+def _lin_no_eps(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    return _lin(kind, L, d, niter, aux, 1.0 if (kind == "reg" and eps) else eps, W, m0, history)
+
+
+def _lin_r_transposed(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    if kind == "reg" and aux is not None:
+        aux = np.asarray(aux).T
+    return _lin(kind, L, d, niter, aux, eps, W, m0, history)
+
+
+def _lin_model_for_update(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    m, res = _lin(kind, L, d, niter, aux, eps, W, m0, history)
+    return (m + m0 if m0 is not None else m), res
+
+
+def _lin_weight_twice(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    W = np.asarray(W) @ np.asarray(W) if W is not None else None
+    return _lin(kind, L, d, niter, aux, eps, W, m0, history)
+
+
+def _lin_weight_transposed(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    W = np.asarray(W).T if W is not None else None
+    return _lin(kind, L, d, niter, aux, eps, W, m0, history)
+
+
+def _lin_prec_eps_on_m(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
+    if kind != "prec":
+        return _lin(kind, L, d, niter, aux, eps, W, m0, history)
+    P = np.asarray(aux)
+    # min |W(L P p - d)|^2 + eps^2 |P p|^2: the damping on m instead of p
+    p, res = _lin("reg", np.asarray(L) @ P, d, niter, P, eps, W, None, history)
+    return P @ p, res
+
+
+LIN_MUTANTS = [
+    ("reg_solver without its eps", _lin_no_eps),
+    ("reg_solver with the regulariser transposed", _lin_r_transposed),
+    ("the model returned in place of the update from m0", _lin_model_for_update),
+    ("the data weight applied twice", _lin_weight_twice),
+    ("the data weight transposed", _lin_weight_transposed),
+    ("prec_solver damping m = P p instead of p", _lin_prec_eps_on_m),
+]
+LIN_PINS = [("test_lin_solve_forms_reach_their_closed_forms", [()])]
+
+
+def test_solver_pins_pass_on_the_oracle():
+    O.lin_solve = _lin
+    try:
+        assert _failing(LIN_PINS) == []
+    finally:
+        O.lin_solve = _LIN
+
+
+@pytest.mark.parametrize("label,fn", LIN_MUTANTS, ids=[m[0] for m in LIN_MUTANTS])
+def test_solver_pins_catch_a_planted_mistake(label, fn):
+    O.lin_solve = fn
+    try:
+        failing = _failing(LIN_PINS)
+    finally:
+        O.lin_solve = _LIN
+    assert failing, f"no closed form caught: {label}"
