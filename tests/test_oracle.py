@@ -662,6 +662,10 @@ def test_tmute_hand_example():
     time before ot starts at 0, one past the trace keeps nothing."""
     s = O.tmute_start(100, 0.0, 0.004, 3, 100.0, 50.0, 0.1, 2000.0)
     assert list(s) == [37, 43, 50]
+    # the reference's float arithmetic (Born/tmute.cpp:63-64, float axis o/d,
+    # t_0 and vmute): 30 m / 1500 m/s = 0.02 s is sample 5 exactly, but in
+    # FP32 0.02f / 0.004f = 4.9999995 truncates to 4 (FP64 would give 5)
+    assert list(O.tmute_start(100, 0.0, 0.004, 4, 0.0, 10.0, 0.0, 1500.0)) == [0, 1, 3, 4]
     assert list(O.tmute_start(10, 0.5, 0.004, 2, 0.0, 10.0, 0.1, 2000.0)) == [0, 0]
     assert list(O.tmute_start(10, 0.0, 0.004, 1, 0.0, 10.0, 5.0, 2000.0)) == [10]
     x = np.arange(2 * 3 * 100, dtype=np.float64).reshape(2, 3, 100)

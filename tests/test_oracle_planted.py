@@ -6,7 +6,8 @@ indexing (closed-form recursions) and the stencil and sponge (Fornberg
 weights, polynomial and plane-wave exactness, the sponge formulas, the VL/LV
 transpose).  CPU only; the copies are built in a temporary directory.  A
 third group swaps mistaken versions of the oracle's numpy operators (D_t,
-D_z, the data window, the wavelet derivative, the receiver mask) into the
+D_z, the data window, the wavelet derivative, the receiver mask, tmute's
+first kept sample) into the
 module and runs their closed forms and hand examples; a fourth wraps the
 solver forms (lin_solve) with assembly mistakes and runs their closed forms."""
 import os
@@ -232,6 +233,35 @@ def _mask_xy_swapped(locs, n):
     return rl.astype(np.int32)
 
 
+def _tmute(conv):
+    def f(nt, ot, dt, nrx, orx, drx, t0, vmute):
+        out = np.zeros(nrx, np.int64)
+        for i in range(nrx):
+            out[i] = conv(nt, ot, dt, orx, drx, t0, vmute, i)
+        return out
+    return f
+
+
+def _tmute_f64(nt, ot, dt, orx, drx, t0, vmute, i):
+    q = (t0 + (orx + drx * i) / vmute - ot) / dt
+    return 0 if q <= 0 else (nt if q >= nt else int(q))
+
+
+def _tmute_f32(nt, ot, dt, orx, drx, t0, vmute, i):
+    f32 = np.float32
+    return float((f32(t0) + (f32(orx) + f32(drx) * f32(i)) / f32(vmute) - f32(ot)) / f32(dt))
+
+
+def _tmute_rounded(nt, ot, dt, orx, drx, t0, vmute, i):
+    q = _tmute_f32(nt, ot, dt, orx, drx, t0, vmute, i)
+    return 0 if q <= 0 else (nt if q >= nt else int(round(q)))
+
+
+def _tmute_unclamped(nt, ot, dt, orx, drx, t0, vmute, i):
+    q = _tmute_f32(nt, ot, dt, orx, drx, t0, vmute, i)
+    return nt if q >= nt else int(q)
+
+
 PY_MUTANTS = [
     ("D_t with the opposite sign",
      {"dt_forward": lambda m, dt: -_ORIG["dt_forward"](m, dt),
@@ -246,13 +276,16 @@ PY_MUTANTS = [
     ("receivers on the last halo cell kept", {"mask_inactive": _mask(4, 3)}),
     ("receivers on the first updated cell dropped", {"mask_inactive": _mask(5, 4)}),
     ("receiver mask with x and y swapped", {"mask_inactive": _mask_xy_swapped}),
+    ("tmute start computed in FP64", {"tmute_start": _tmute(_tmute_f64)}),
+    ("tmute start rounded, not truncated", {"tmute_start": _tmute(_tmute_rounded)}),
+    ("tmute start not clamped at 0", {"tmute_start": _tmute(_tmute_unclamped)}),
 ]
 _ORIG = {name: getattr(O, name) for m in PY_MUTANTS for name in m[1]}
 
 PY_PINS = [(name, [()]) for name in (
     "test_dz_dt_window_are_adjoint_pairs", "test_dz_and_dt_closed_forms",
     "test_window_hand_example", "test_wavelet_derivative_hand_example",
-    "test_inactive_receivers_hand_example")]
+    "test_inactive_receivers_hand_example", "test_tmute_hand_example")]
 
 
 def test_numpy_pins_pass_on_the_oracle():
