@@ -9,7 +9,7 @@
  *                  (Born/combo_oper.cpp:1-282), and caller-defined
  *                  operators through callbacks;
  *   brtm_solver -- simple_solver, reg_solver and prec_solver
- *                  (Born/lin_solver.cpp:7-53,79-101,115-160) driven by
+ *                  (Born/lin_solver.cpp:7-53,79-101,115-155) driven by
  *                  cgstep (Born/cgstep.cpp:14-58) or flexible CG.
  *
  * Stacked vectors (the reference's super_vector, Born/super_vector.cpp)

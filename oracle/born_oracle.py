@@ -17,16 +17,23 @@ operators that surround the propagation in the reference:
 * ``sinc_table``              -- Born/sinc_bob.cpp:4-114 (Hale's
   least-squares tapered sinc, Levinson solve) in FP64;
 * ``cgstep``                  -- Born/cgstep.cpp:14-58;
+* ``fcg_solve``               -- flexible CG with truncated history (cgstep's
+  plane search at history 1);
 * ``lin_solve``               -- simple_solver / reg_solver / prec_solver,
-  Born/lin_solver.cpp:7-53,79-160 (stacked systems of Born/combo_oper.cpp);
+  Born/lin_solver.cpp:7-53,79-155 (stacked systems of Born/combo_oper.cpp);
 * ``tmute_start/tmute_apply`` -- Born/tmute.cpp:4-76;
-* ``source_geometry``         -- Born/source_func_3d.cpp:100-133;
-* ``receiver_index``          -- Born/map_data_3d.cpp:4-32.
+* ``source_geometry``, ``wavelet_derivative`` -- Born/source_func_3d.cpp:
+  100-133 and 9-13;
+* ``receiver_index``, ``mask_inactive`` -- Born/map_data_3d.cpp:4-32, and
+  receivers outside the update set marked inactive;
+* ``CompositeBorn``           -- the multi-shot operator D_t W B D_z E of
+  Born/rtm_zero_op_3d.cpp:87-293 over the C propagation.
 
 Every function here is pinned by tests/test_oracle.py against a closed form,
-a library routine (scipy Toeplitz solve, numpy lstsq), the reference's own
-code built from its sources into oracle/_ref (sinc_bob, cgstep), or an
-adjoint identity.
+a hand example, a library routine (scipy Toeplitz solve, numpy lstsq), the
+reference's own code built from its sources into oracle/_ref (sinc_bob,
+cgstep), or an adjoint identity; tests/test_oracle_planted.py checks that
+plausible mistakes planted in copies of the oracle fail those pins.
 """
 from __future__ import annotations
 
@@ -409,7 +416,7 @@ def _ops(A):
 
 def lin_solve(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
     """simple_solver / reg_solver / prec_solver (Born/lin_solver.cpp:7-53,
-    79-101,115-160) in FP64, on operators given as dense matrices or
+    79-101,115-155) in FP64, on operators given as dense matrices or
     (forward, adjoint) pairs:
 
     * "simple": min |W(L m - d)|;
