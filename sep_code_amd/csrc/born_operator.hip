@@ -274,7 +274,7 @@ int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out) {
   op->hold.reset(ctx);
   op->ctx = ctx;
   op->d = d;
-  op->nt_fd = (d.nt_data - 1) * d.jt + 1;  // Born/rtm_zero_op_3d.cpp:23
+  op->nt_fd = (d.nt_data - 1) * d.jt + 1;  // Born/rtm_zero_op_3d.cpp:21
   op->ntr_d = d.nt_data + 8;               // Born/rtm_zero_op_3d.cpp:99
   op->src_locs.assign(d.src_locs, d.src_locs + (size_t)d.nshots * d.nsrc);
   op->src_traces.assign(d.src_traces, d.src_traces + (size_t)d.nshots * d.nsrc * d.ntr_s);

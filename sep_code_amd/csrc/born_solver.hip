@@ -283,7 +283,7 @@ int gram_solve(int K, const double *M, const double *b, double *x, double tol = 
   return kept;
 }
 
-// One iteration of lin_solver::solve (Born/lin_solver.cpp:121-137):
+// One iteration of lin_solver::solve (Born/lin_solver.cpp:123-141):
 // g = A' rr, direction = P(g) (or g), gg = A direction, then the step.
 // Returns BRTM_OK with *stop set on a breakdown.
 int iterate(brtm_solver *S, const brtm_cg_params *prm, brtm_cg_stats *st, double *alfa_out,
