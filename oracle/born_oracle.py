@@ -469,7 +469,7 @@ def lin_solve(kind, L, d, niter, aux=None, eps=0.0, W=None, m0=None, history=1):
 
 def tmute_start(nt, ot, dt, nrx, orx, drx, t0, vmute):
     """First kept sample of each receiver column of tmute (Born/tmute.cpp:
-    57-59): float t0 + (o2 + d2*i2)/vmute, truncated (t - o1)/d1, clamped to
+    28-29, 63-64): float t0 + (o2 + d2*i2)/vmute, truncated (t - o1)/d1, clamped to
     [0, nt] (the reference indexes from a negative start when the mute time
     precedes o1)."""
     f32 = np.float32

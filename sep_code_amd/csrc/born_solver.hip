@@ -436,7 +436,7 @@ int brtm_linop_tmute(brtm_ctx *ctx, int nt, float ot, float dt, int nrx, float o
   BRTM_REQUIRE(ctx && out, BRTM_EINVAL, "NULL argument");
   BRTM_REQUIRE(nt > 0 && nrx > 0 && nouter > 0 && dt > 0 && vmute != 0, BRTM_EINVAL,
                "bad tmute geometry");
-  // Born/tmute.cpp:57-59: float t0 = t_0 + (o2 + d2*i2)/vmute; the first
+  // Born/tmute.cpp:28-29,63-64: float t0 = t_0 + (o2 + d2*i2)/vmute; the first
   // kept sample is (int)((t0 - o1)/d1) (truncation), clamped to [0, nt].
   // One rounding per statement (no fused multiply-add), as the reference's
   // host build computes it.
@@ -625,7 +625,7 @@ int brtm_solver_create(int kind, brtm_linop *op, const brtm_vec *data, brtm_lino
     return rc;
   };
   int rc = BRTM_OK;
-  // create_wt_op (Born/lin_solver.cpp:69-77): W o L
+  // create_wt_op (Born/lin_solver.cpp:70-78): W o L
   brtm_linop *wt_op = op;
   if (wt) {
     brtm_linop *parts[2] = {op, wt};
@@ -695,7 +695,7 @@ int brtm_solver_create(int kind, brtm_linop *op, const brtm_vec *data, brtm_lino
     brtm_vec_destroy(rtemp);
     if (rc != BRTM_OK) return fail(rc);
   }
-  // create_solver_vecs (Born/lin_solver.cpp:61-67)
+  // create_solver_vecs (Born/lin_solver.cpp:62-69)
   rc = brtm_vec_create(c, S->oper->nd, &S->m);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, S->oper->nd, &S->g);
   if (rc == BRTM_OK) rc = brtm_vec_create(c, S->oper->nd, &S->s);
