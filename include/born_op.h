@@ -42,8 +42,11 @@ typedef struct brtm_op brtm_op;
 int brtm_vec_create(brtm_ctx *ctx, long n, brtm_vec **out);
 int brtm_vec_destroy(brtm_vec *v);
 long brtm_vec_size(const brtm_vec *v);
-/* Device address of the vector's FP32 storage (for kernels of the caller). */
+/* Device address of the vector's FP32 storage (for kernels of the caller;
+ * hypercube_float::vals, Born/hypercube_float.h). */
 void *brtm_vec_data(brtm_vec *v);
+/* Host <-> device copies of the values (no counterpart: the reference's
+ * vectors live in host memory). */
 int brtm_vec_upload(brtm_vec *v, const float *host);
 int brtm_vec_download(const brtm_vec *v, float *host);
 int brtm_vec_zero(brtm_vec *v);                                 /* set_val(0)   */
@@ -53,7 +56,8 @@ int brtm_vec_scale(brtm_vec *v, double s);                       /* scale       
 int brtm_vec_scale_add(brtm_vec *v, double a, const brtm_vec *x, double b);
 int brtm_vec_add(brtm_vec *v, const brtm_vec *x);                /* add          */
 int brtm_vec_mult(brtm_vec *v, const brtm_vec *x);               /* mult         */
-/* FP64 inner product, deterministic (fixed-grid reduction). */
+/* FP64 inner product, deterministic (fixed-grid reduction): my_vector::dot
+ * (Born/hypercube_float.cpp:123-132). */
 int brtm_vec_dot(const brtm_vec *a, const brtm_vec *b, double *out);
 /* Uniform values in [-0.5, 0.5) from SplitMix64(seed)
  * (my_vector::random, Born/hypercube_float.cpp:118-122, with a seeded
@@ -83,8 +87,9 @@ typedef struct {
 /* Replaces rtm_zero_op::rtm_zero_op (Born/rtm_zero_op_3d.cpp:8-37): copies
  * the geometry; the context must already hold the space and velocity. */
 int brtm_op_create(brtm_ctx *ctx, const brtm_op_desc *desc, brtm_op **out);
-/* Releases the per-shot stores this operator's brtm_op_prepare made (not
- * those another operator on the context has prepared since). */
+/* Replaces ~rtm_zero_op (Born/rtm_zero_op_3d.h:20): releases the per-shot
+ * stores this operator's brtm_op_prepare made (not those another operator
+ * on the context has prepared since). */
 int brtm_op_destroy(brtm_op *op);
 /* Replaces create_source_fields (Born/rtm_zero_op_3d.cpp:38-85): propagate
  * and store every shot's source wavefield.  The stores are the context's,

@@ -54,7 +54,7 @@ typedef struct brtm_ctx brtm_ctx;
 /* Message describing the last failing call on this thread. */
 const char *brtm_last_error(void);
 
-/* Library version string. */
+/* Library version string (no counterpart in the reference). */
 const char *brtm_version(void);
 
 /* Replaces setup_cuda (Born/gpu_funcs_3d.cu:10-45): bind a context to one
@@ -97,7 +97,8 @@ int brtm_set_source(brtm_ctx *ctx, int npts, int ntr, const int *locs, const flo
  * and their traces [ny][nx][ntr] (vals may be NULL: zero traces). */
 int brtm_set_receivers(brtm_ctx *ctx, int nx, int ny, int ntr, const int *locs,
                        const float *vals);
-/* Download the receiver traces in the layout [ny][nx][ntr]. */
+/* Download the receiver traces in the layout [ny][nx][ntr] (the copy back
+ * at the end of rtm_forward, Born/gpu_funcs_3d.cu:489). */
 int brtm_get_receivers(brtm_ctx *ctx, float *vals);
 
 /* Born model (compute grid, the img argument of rtm_forward,
@@ -107,8 +108,10 @@ int brtm_set_model(brtm_ctx *ctx, const float *m);
 int brtm_get_image(brtm_ctx *ctx, float *img);
 int brtm_zero_image(brtm_ctx *ctx);
 
-/* Source-wavefield stores are kept per shot; subsequent propagation calls
- * use the store of the selected shot. */
+/* Source-wavefield stores are kept per shot in HBM; subsequent propagation
+ * calls use the store of the selected shot.  They replace the last two
+ * source fields that rtm_zero_op::create_source_fields writes per shot to
+ * its source_fields file (Born/rtm_zero_op_3d.cpp:70-76). */
 int brtm_select_shot(brtm_ctx *ctx, int ishot);
 int brtm_release_store(brtm_ctx *ctx, int ishot);
 
@@ -140,6 +143,9 @@ int brtm_rtm_adjoint(brtm_ctx *ctx, int jt, int nt, int jimg);
 
 /* Imaging scale (1e-6 in the reference, Born/gpu_funcs_3d.cu:117,128). */
 int brtm_set_imaging_scale(brtm_ctx *ctx, float cimg);
+
+/* ---- Diagnostics, timing and tuning (tests and benchmarks): no
+ * counterpart in the reference, and none changes a result's meaning. ---- */
 
 /* Decode snapshot k of the selected shot's store into host FP32. */
 int brtm_get_snapshot(brtm_ctx *ctx, int k, float *out);
@@ -231,7 +237,8 @@ int brtm_set_two_step(brtm_ctx *ctx, int mode);
 /* Calibration: nrep launches of a plain streaming copy of `bytes` bytes
  * (16-byte vector loads and stores) between two device buffers; avg_ms
  * receives the mean launch duration.  Its rocprof FETCH_SIZE / WRITE_SIZE
- * scale the counters of the other kernels to HBM bytes. */
+ * scale the counters of the other kernels to HBM bytes.  No counterpart in
+ * the reference. */
 int brtm_copy_bench(brtm_ctx *ctx, double bytes, int nrep, double *avg_ms);
 
 /* Calibration: nrep launches of out = sum of nin (1..3) input streams of
@@ -257,10 +264,12 @@ int brtm_stream_bench(brtm_ctx *ctx, int nin, double bytes, int nt_store, int nr
  * different streams) share that array and race: serialise them. */
 int brtm_dot_noalloc(brtm_ctx *ctx, const float *a, const float *b, long n, double *out);
 
-/* Device synchronisation of the library stream. */
+/* Device synchronisation of the library stream (no counterpart in the
+ * reference, whose calls return after their last copy). */
 int brtm_sync(brtm_ctx *ctx);
 
-/* Bytes of device memory the context currently holds. */
+/* Bytes of device memory the context currently holds (no counterpart in
+ * the reference). */
 int brtm_device_bytes(brtm_ctx *ctx, double *bytes);
 
 #ifdef __cplusplus

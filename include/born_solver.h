@@ -41,7 +41,9 @@ int brtm_linop_born(brtm_op *op, brtm_linop **out);
 /* i_op (Born/i_op.cpp:4-28): out = add*out + scale*in on n elements. */
 int brtm_linop_identity(brtm_ctx *ctx, long n, brtm_linop **out);
 /* Pointwise weight: out = add*out + scale*(w . in); self-adjoint.  The
- * weight vector is borrowed. */
+ * weight vector is borrowed.  No class of its own in the reference, whose
+ * data weight (wt) is any my_operator (Born/my_operator.h:7-36); this is
+ * the diagonal one the preconditioned and weighted solves use. */
 int brtm_linop_diagonal(const brtm_vec *w, brtm_linop **out);
 /* tmute (Born/tmute.cpp:4-76): data [nouter][nrx][nt] (t fastest; nouter =
  * nry*nshots) keep sample it of trace rx when it >= (int)((t0 + (orx +
