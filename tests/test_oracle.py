@@ -156,6 +156,65 @@ def test_undamped_leapfrog_is_time_reversible():
     np.testing.assert_allclose(back_old, p1, atol=1e-9)
 
 
+def test_leapfrog_stability_limit_from_the_weights():
+    """The 3-D 8th-order leapfrog limit.  The stencil's largest eigenvalue
+    magnitude is its checkerboard (Nyquist) symbol
+    lam = -(c0 + 2 sum_{k,a} (-1)^k c_{k,a}), and p' = 2p - p_old + V L p is
+    stable while V lam <= 4.  For equal spacing that is v dt / d = 0.4529,
+    the limit the library refuses V beyond.  The undamped oracle leapfrog
+    started from a checkerboard stays bounded over 300 steps at 0.98 of the
+    limit and blows up at 1.02 (unequal spacing)."""
+    def lam_of(d):
+        c = O.coeffs(*d)
+        return -(c[0] + 2 * sum(c[1 + 3 * (k - 1) + a] * (-1) ** k
+                                for k in range(1, 5) for a in range(3)))
+
+    assert abs(math.sqrt(4 / lam_of((10.0, 10.0, 10.0))) / 10.0 - 0.45286) < 1e-5
+    d = (10.0, 12.0, 14.0)
+    lam = lam_of(d)
+    n = (24, 26, 28)
+    i = np.indices((n[2], n[1], n[0]))
+    inside = np.zeros((n[2], n[1], n[0]))
+    inside[4:-4, 4:-4, 4:-4] = 1
+    p0 = ((-1.0) ** (i[0] + i[1] + i[2]) * inside).ravel()
+    peak = {}
+    for f in (0.98, 1.02):
+        P = O.Problem(n, d, np.full((n[2], n[1], n[0]), f * 4 / lam), 0.0, 0.0)
+        cur, _ = P.propagate(0, 300, p0.copy(), p0.copy())
+        peak[f] = np.abs(cur).max()
+    assert peak[0.98] < 50 and peak[1.02] > 1e6
+
+
+def test_undamped_leapfrog_conserves_discrete_energy():
+    """Without the sponge the VL step is p' = 2p - p_old + V L p with L
+    symmetric on the update set, which conserves
+    E = <V^-1 (p' - p), p' - p> - <L p', p> exactly (variable V, unequal
+    spacing): a non-symmetric tap, damping, or V applied inside L breaks it."""
+    r = rng(3)
+    n, d = (20, 22, 24), (10.0, 12.0, 14.0)
+    shape = (n[2], n[1], n[0])
+    V = r.uniform(0.05, 0.12, shape) * min(d) ** 2 / 4.0
+    P = O.Problem(n, d, V, 0.0, 0.0)
+    P1 = O.Problem(n, d, np.ones(shape), 0.0, 0.0)
+    inside = np.zeros(shape)
+    inside[4:-4, 4:-4, 4:-4] = 1
+    inside = inside.ravel()
+    Vi = np.where(inside > 0, 1.0 / V.ravel(), 0.0)
+
+    def lap(p):  # V = 1, no sponge: step(p, 0) = L p + 2p
+        return P1.step_vl(p, np.zeros_like(p)) - 2 * p
+
+    p_old = r.standard_normal(inside.size) * inside
+    p = r.standard_normal(inside.size) * inside
+    E = []
+    for _ in range(100):
+        pn = P.step_vl(p, p_old)
+        E.append((pn - p) @ (Vi * (pn - p)) - lap(pn) @ p)
+        p_old, p = p, pn
+    E = np.array(E)
+    assert np.abs(E - E[0]).max() < 1e-12 * abs(E[0])
+
+
 def test_sponge_dissipates_energy():
     P, r = small_problem(n=(32, 32, 32), bc_a=8.0, bc_b=0.05)
     p = np.zeros(P.ncell)

@@ -78,7 +78,8 @@ STENCIL_PINS = [(name, [()]) for name in (
     "test_stencil_laplacian_of_polynomial_anisotropic", "test_stencil_symbol_on_plane_wave",
     "test_sponge_table_closed_form", "test_sponge_edge_distance_matches_reference_formula",
     "test_lv_step_is_transpose_of_vl_step", "test_undamped_leapfrog_is_time_reversible",
-    "test_sponge_dissipates_energy")]
+    "test_sponge_dissipates_energy", "test_leapfrog_stability_limit_from_the_weights",
+    "test_undamped_leapfrog_conserves_discrete_energy")]
 
 CLOSED_FORM = [
     ("test_source_prop_one_sample_delay_jt1", [()]),
