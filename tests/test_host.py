@@ -226,3 +226,17 @@ def test_bench_cpu_full_shot_matches_fp64_oracle():
         r = bench.cpu_full_shot(w, D, ref, runs=1)
         assert r["image_rel_diff_vs_gpu"] < 1e-5, (store, r)
         assert r["value"] > 0 and r["kind"] == "port"
+
+
+def test_drop_in_header_has_the_reference_signatures():
+    """include/gpu_funcs_3d.h against the reference's own header
+    (Born/gpu_funcs_3d.h:5-27): oracle/Makefile's abi target compiles both in
+    one C++ translation unit, where a differing parameter type is a
+    conflicting extern "C" declaration."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "abi"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    if "reference absent" in r.stdout:
+        pytest.skip("the reference's sources are not in this container")
+    assert "drop-in signatures agree" in r.stdout
