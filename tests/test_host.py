@@ -240,3 +240,36 @@ def test_drop_in_header_has_the_reference_signatures():
     if "reference absent" in r.stdout:
         pytest.skip("the reference's sources are not in this container")
     assert "drop-in signatures agree" in r.stdout
+
+
+def test_workloads_match_their_fingerprints():
+    """The synthetic workloads the measurements are quoted on (128^3 and
+    512^3 RTM shots, the 256^3 LSRTM problem for seeds 77 and 78) are the
+    ones tests/golden/workloads.json recorded: sizes and source/receiver
+    cells exactly, velocity, reflectivity and source traces to 1e-6."""
+    import importlib.util
+    import json
+    path = os.path.join(ROOT, "tests", "golden", "make_workload_fixtures.py")
+    spec = importlib.util.spec_from_file_location("make_workload_fixtures", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        want = json.load(f)
+    assert set(want) == set(gen.WORKLOADS)
+
+    def same(a, b, where):
+        if isinstance(a, dict):
+            assert set(a) == set(b), where
+            for k in a:
+                same(a[k], b[k], f"{where}.{k}")
+        elif isinstance(a, list):
+            assert len(a) == len(b), where
+            for i, (x, y) in enumerate(zip(a, b)):
+                same(x, y, f"{where}[{i}]")
+        elif isinstance(a, float):
+            assert abs(a - b) <= 1e-6 * max(abs(a), abs(b)) + 1e-30, (where, a, b)
+        else:
+            assert a == b, (where, a, b)
+
+    for name, make in gen.WORKLOADS.items():
+        same(json.loads(json.dumps(gen.fingerprint(make()))), want[name], name)
