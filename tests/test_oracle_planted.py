@@ -1,15 +1,19 @@
-"""The oracle's pins (tests/test_oracle.py) catch the plausible mistakes:
-each is planted in a copy of the oracle's C source (oracle/born_oracle.c),
-built into a separate FP64 library, and the pinning tests run against it
-must fail, while they pass on the oracle itself.  Two groups: the time
-indexing (closed-form recursions) and the stencil and sponge (Fornberg
-weights, polynomial and plane-wave exactness, the sponge formulas, the VL/LV
-transpose).  CPU only; the copies are built in a temporary directory.  A
-third group swaps mistaken versions of the oracle's numpy operators (D_t,
-D_z, the data window, the wavelet derivative, the receiver mask, tmute's
-first kept sample) into the
-module and runs their closed forms and hand examples; a fourth wraps the
-solver forms (lin_solve) with assembly mistakes and runs their closed forms."""
+"""The oracle's pins (tests/test_oracle.py) catch plausible mistakes: each
+mistake is planted in a copy of the oracle, the pinning tests run against
+the copy must fail, and they pass on the oracle itself.  CPU only.  Four
+groups:
+
+* time indexing (closed-form recursions) and
+* stencil and sponge (Fornberg weights, polynomial and plane-wave
+  exactness, the sponge formulas, the VL/LV transpose, the stability limit,
+  energy conservation): planted in copies of oracle/born_oracle.c, each built
+  into its own FP64 library in a temporary directory;
+* the numpy operators (D_t, D_z, the data window, the wavelet derivative,
+  the receiver mask, tmute's first kept sample): mistaken versions swapped
+  into the module, against their closed forms and hand examples;
+* the solver forms (lin_solve) wrapped with assembly mistakes, against their
+  closed forms.
+"""
 import os
 import shutil
 import subprocess
