@@ -15,6 +15,7 @@
 
 #include "../../include/born_solver.h"
 #include "born_ctx.h"
+#include "gram_solve.h"
 
 struct brtm_linop {
   brtm::CtxHold hold;
@@ -236,51 +237,6 @@ int check_parts(int n, brtm_linop *const *ops) {
                  "operators of a combination must share a context");
   }
   return BRTM_OK;
-}
-
-// Solve the K x K symmetric system M x = b (FP64) by Cholesky, skipping
-// basis vectors whose pivot falls below tol times their own norm (nearly
-// dependent on the ones before them): their coefficient is 0.  Returns the
-// number of basis vectors kept (0 when the first one has no length).
-int gram_solve(int K, const double *M, const double *b, double *x, double tol = 1e-9) {
-  double Lm[BRTM_CG_MAX_HISTORY + 1][BRTM_CG_MAX_HISTORY + 1] = {};
-  bool keep[BRTM_CG_MAX_HISTORY + 1] = {};
-  int kept = 0;
-  for (int i = 0; i < K; ++i) {
-    double d = M[i * K + i];
-    for (int k = 0; k < i; ++k)
-      if (keep[k]) d -= Lm[i][k] * Lm[i][k];
-    if (!(d > tol * M[i * K + i]) || !(M[i * K + i] > 0)) {
-      keep[i] = false;
-      continue;
-    }
-    keep[i] = true;
-    ++kept;
-    Lm[i][i] = std::sqrt(d);
-    for (int j = i + 1; j < K; ++j) {
-      double v = M[j * K + i];
-      for (int k = 0; k < i; ++k)
-        if (keep[k]) v -= Lm[j][k] * Lm[i][k];
-      Lm[j][i] = v / Lm[i][i];
-    }
-  }
-  double y[BRTM_CG_MAX_HISTORY + 1] = {};
-  for (int i = 0; i < K; ++i) {
-    if (!keep[i]) continue;
-    double v = b[i];
-    for (int k = 0; k < i; ++k)
-      if (keep[k]) v -= Lm[i][k] * y[k];
-    y[i] = v / Lm[i][i];
-  }
-  for (int i = K - 1; i >= 0; --i) {
-    x[i] = 0;
-    if (!keep[i]) continue;
-    double v = y[i];
-    for (int k = i + 1; k < K; ++k)
-      if (keep[k]) v -= Lm[k][i] * x[k];
-    x[i] = v / Lm[i][i];
-  }
-  return kept;
 }
 
 // One iteration of lin_solver::solve (Born/lin_solver.cpp:123-141):

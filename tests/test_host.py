@@ -273,3 +273,60 @@ def test_workloads_match_their_fingerprints():
 
     for name, make in gen.WORKLOADS.items():
         same(json.loads(json.dumps(gen.fingerprint(make()))), want[name], name)
+
+
+def test_gram_solve_drops_dependent_directions(tmp_path):
+    """The flexible CG's host-side Gram solve (sep_code_amd/csrc/gram_solve.h,
+    built here with g++ on its own): on a full-rank K x K Gram matrix
+    (K up to BRTM_CG_MAX_HISTORY + 1) it solves M x = b; a direction that
+    depends on earlier ones (exactly, or to a pivot below tol) gets x = 0 and
+    the rest is the least-squares solution over the others; a direction of
+    zero length is dropped the same way, and with none left nothing is kept."""
+    import subprocess
+    src = tmp_path / "gram.cpp"
+    src.write_text('#include "gram_solve.h"\n'
+                   'extern "C" int gram(int K, const double *M, const double *b, double *x,'
+                   ' double tol) { return brtm::gram_solve(K, M, b, x, tol); }\n')
+    so = tmp_path / "libgram.so"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-shared", "-fPIC",
+                    "-I", os.path.join(ROOT, "sep_code_amd", "csrc"), "-o", str(so), str(src)],
+                   check=True)
+    lib = ctypes.CDLL(str(so))
+    P = ctypes.POINTER(ctypes.c_double)
+    lib.gram.argtypes = [ctypes.c_int, P, P, P, ctypes.c_double]
+
+    def solve(B, t, tol=1e-9):
+        M = np.ascontiguousarray(B.T @ B)
+        b = np.ascontiguousarray(B.T @ t)
+        x = np.zeros(B.shape[1])
+        kept = lib.gram(B.shape[1], M.ctypes.data_as(P), b.ctypes.data_as(P),
+                        x.ctypes.data_as(P), tol)
+        return kept, x
+
+    r = np.random.default_rng(40)
+    for K in (1, 2, 5, 9, 17):
+        B, t = r.standard_normal((60, K)), r.standard_normal(60)
+        kept, x = solve(B, t)
+        assert kept == K
+        np.testing.assert_allclose(x, np.linalg.solve(B.T @ B, B.T @ t), rtol=1e-9, atol=1e-12)
+    # column 2 = column 0 - 2 column 1: dropped, the others solve least squares
+    B, t = r.standard_normal((40, 6)), r.standard_normal(40)
+    B[:, 2] = B[:, 0] - 2 * B[:, 1]
+    kept, x = solve(B, t)
+    keep = [0, 1, 3, 4, 5]
+    assert kept == 5 and x[2] == 0
+    np.testing.assert_allclose(x[keep], np.linalg.lstsq(B[:, keep], t, rcond=None)[0],
+                               rtol=1e-8, atol=1e-10)
+    # nearly dependent (relative pivot 1e-12 < tol): dropped; with tol below it: kept
+    B[:, 2] += 1e-6 * r.standard_normal(40) * np.linalg.norm(B[:, 2]) / np.sqrt(40)
+    assert solve(B, t)[0] == 5
+    assert solve(B, t, tol=1e-14)[0] == 6
+    # a direction of zero length is dropped; with every one of zero length
+    # nothing is kept
+    B = r.standard_normal((40, 4))
+    B[:, 0] = 0
+    kept, x = solve(B, t)
+    assert kept == 3 and x[0] == 0
+    np.testing.assert_allclose(x[1:], np.linalg.lstsq(B[:, 1:], t, rcond=None)[0],
+                               rtol=1e-8, atol=1e-10)
+    assert solve(np.zeros((40, 3)), t)[0] == 0
