@@ -16,6 +16,7 @@
 #include "../../include/born_solver.h"
 #include "born_ctx.h"
 #include "gram_solve.h"
+#include "tmute_start.h"
 
 struct brtm_linop {
   brtm::CtxHold hold;
@@ -392,20 +393,8 @@ int brtm_linop_tmute(brtm_ctx *ctx, int nt, float ot, float dt, int nrx, float o
   BRTM_REQUIRE(ctx && out, BRTM_EINVAL, "NULL argument");
   BRTM_REQUIRE(nt > 0 && nrx > 0 && nouter > 0 && dt > 0 && vmute != 0, BRTM_EINVAL,
                "bad tmute geometry");
-  // Born/tmute.cpp:28-29,63-64: float t0 = t_0 + (o2 + d2*i2)/vmute; the first
-  // kept sample is (int)((t0 - o1)/d1) (truncation), clamped to [0, nt].
-  // One rounding per statement (no fused multiply-add), as the reference's
-  // host build computes it.
   std::vector<int> start(nrx);
-  for (int i = 0; i < nrx; ++i) {
-    const float x = drx * (float)i;
-    const float xo = orx + x;
-    const float q = xo / vmute;
-    const float t = t0 + q;
-    const float u = t - ot;
-    const float f = u / dt;
-    start[i] = f <= 0.f ? 0 : (f >= (float)nt ? nt : (int)f);
-  }
+  tmute_starts(nt, ot, dt, nrx, orx, drx, t0, vmute, start.data());
   brtm_linop *A = nullptr;
   LS_TRY(new_linop(ctx, brtm_linop::TMUTE, nouter * nrx * (long)nt, nouter * nrx * (long)nt, &A));
   A->nt = nt;

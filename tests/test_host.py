@@ -330,3 +330,43 @@ def test_gram_solve_drops_dependent_directions(tmp_path):
     np.testing.assert_allclose(x[1:], np.linalg.lstsq(B[:, 1:], t, rcond=None)[0],
                                rtol=1e-8, atol=1e-10)
     assert solve(np.zeros((40, 3)), t)[0] == 0
+
+
+def test_tmute_starts_follow_the_reference_float_arithmetic(tmp_path):
+    """The library's first kept sample of each receiver column of tmute
+    (sep_code_amd/csrc/tmute_start.h, built here with g++ on its own) equals
+    the oracle's restatement of Born/tmute.cpp:28-29,63-64 on random
+    geometries and on a case where FP32 and FP64 truncate to different
+    samples (30 m at 1500 m/s with 4 ms samples: 4, not 5)."""
+    import subprocess
+    src = tmp_path / "tm.cpp"
+    src.write_text('#include "tmute_start.h"\n'
+                   'extern "C" void starts(int nt, float ot, float dt, int nrx, float orx,'
+                   ' float drx, float t0, float vmute, int *s) {'
+                   ' brtm::tmute_starts(nt, ot, dt, nrx, orx, drx, t0, vmute, s); }\n')
+    so = tmp_path / "libtm.so"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-shared", "-fPIC",
+                    "-I", os.path.join(ROOT, "sep_code_amd", "csrc"), "-o", str(so), str(src)],
+                   check=True)
+    lib = ctypes.CDLL(str(so))
+    F, I = ctypes.c_float, ctypes.c_int
+    lib.starts.argtypes = [I, F, F, I, F, F, F, F, ctypes.POINTER(I)]
+
+    def starts(nt, ot, dt, nrx, orx, drx, t0, vmute):
+        s = (I * nrx)()
+        lib.starts(nt, ot, dt, nrx, orx, drx, t0, vmute, s)
+        return list(s)
+
+    cases = [(100, 0.0, 0.004, 4, 0.0, 10.0, 0.0, 1500.0),
+             (100, 0.0, 0.004, 3, 100.0, 50.0, 0.1, 2000.0),
+             (10, 0.5, 0.004, 2, 0.0, 10.0, 0.1, 2000.0),
+             (10, 0.0, 0.004, 1, 0.0, 10.0, 5.0, 2000.0)]
+    r = np.random.default_rng(12)
+    for _ in range(300):
+        cases.append((int(r.integers(1, 3000)), float(r.uniform(-0.1, 0.2)),
+                      float(r.choice([0.001, 0.002, 0.004, 0.008])), int(r.integers(1, 40)),
+                      float(r.uniform(-500, 500)), float(r.choice([5.0, 10.0, 12.5, 25.0, 50.0])),
+                      float(r.uniform(0, 0.5)), float(r.uniform(1400, 5000))))
+    assert starts(*cases[0]) == [0, 1, 3, 4]
+    for c in cases:
+        assert starts(*c) == [int(v) for v in O.tmute_start(*c)], c
