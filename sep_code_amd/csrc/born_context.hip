@@ -14,6 +14,7 @@
 
 #include "../../include/born_rtm.h"
 #include "born_ctx.h"
+#include "step_tables.h"
 
 // Scratch of the allocation-free FP64 dot product (brtm_dot_noalloc): the
 // kDotBlocks per-block partial sums, then the result.  One copy per GPU,
@@ -846,20 +847,8 @@ int brtm_create_space(brtm_ctx *c, float d1, float d2, float d3, float bc_a, flo
   c->d[2] = d3;
   c->bc_a = bc_a;
   c->bc_b = bc_b;
-  // Born/wave_fkernel.3d8o.cu:26-27 evaluated in double.
-  const double C[3] = {8.0 / (5.0 * (double)d1 * d1), 8.0 / (5.0 * (double)d2 * d2),
-                       8.0 / (5.0 * (double)d3 * d3)};
-  const double f[4] = {1.0, -1.0 / 8.0, 1.0 / 63.0, -1.0 / 896.0};
-  c->coeffs[0] = (float)(-1025.0 / 576.0 * (C[0] + C[1] + C[2]));
-  for (int k = 0; k < 4; ++k)
-    for (int a = 0; a < 3; ++a) c->coeffs[1 + 3 * k + a] = (float)(C[a] * f[k]);
-  // Sponge weights in FP64 (Born/wave_fkernel.3d8o.cu:201-202).
-  c->nw = (int)std::ceil((double)bc_a);
-  c->wtab.assign(std::max(c->nw, 1), 1.f);
-  for (int e = 0; e < c->nw; ++e) {
-    double w = std::exp(-(double)bc_b * ((double)bc_a - e));
-    c->wtab[e] = (float)(w < 1.0 ? w : 1.0);
-  }
+  stencil_coeffs(d1, d2, d3, c->coeffs);
+  sponge_weights(bc_a, bc_b, &c->nw, &c->wtab);
   const size_t fbytes = sizeof(float) * (size_t)c->ncell;
   BRTM_TRY(c->d_wtab.alloc(sizeof(float) * c->wtab.size()));
   BRTM_HIP(h2d(c, c->d_wtab.ptr, c->wtab.data(), sizeof(float) * c->wtab.size()));

@@ -370,3 +370,42 @@ def test_tmute_starts_follow_the_reference_float_arithmetic(tmp_path):
     assert starts(*cases[0]) == [0, 1, 3, 4]
     for c in cases:
         assert starts(*c) == [int(v) for v in O.tmute_start(*c)], c
+
+
+def test_step_tables_equal_the_oracle_bit_for_bit(tmp_path):
+    """The library's stencil weights and sponge weights
+    (sep_code_amd/csrc/step_tables.h, built here with g++ on its own) equal
+    the oracle's FP32 tables (Born/wave_fkernel.3d8o.cu:26-27, 201-202) bit
+    for bit, over random spacings and sponge parameters."""
+    import subprocess
+    src = tmp_path / "tab.cpp"
+    src.write_text('#include "step_tables.h"\n'
+                   'extern "C" void coeffs(float d1, float d2, float d3, float *c) {'
+                   ' brtm::stencil_coeffs(d1, d2, d3, c); }\n'
+                   'extern "C" int sponge(float a, float b, float *w, int cap) {'
+                   ' int nw; std::vector<float> t; brtm::sponge_weights(a, b, &nw, &t);'
+                   ' for (int i = 0; i < (int)t.size() && i < cap; ++i) w[i] = t[i];'
+                   ' return (int)t.size() == std::max(nw, 1) ? nw : -1; }\n')
+    so = tmp_path / "libtab.so"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-shared", "-fPIC",
+                    "-I", os.path.join(ROOT, "sep_code_amd", "csrc"), "-o", str(so), str(src)],
+                   check=True)
+    lib = ctypes.CDLL(str(so))
+    F, P = ctypes.c_float, ctypes.c_void_p
+    lib.coeffs.argtypes = [F, F, F, P]
+    lib.sponge.argtypes = [F, F, P, ctypes.c_int]
+    lib.sponge.restype = ctypes.c_int
+    r = np.random.default_rng(21)
+    for _ in range(200):
+        d = [float(np.float32(x)) for x in r.uniform(1.0, 50.0, 3)]
+        c = np.zeros(13, np.float32)
+        lib.coeffs(*d, c.ctypes.data)
+        np.testing.assert_array_equal(c, O.coeffs(*d, prec=32))
+        a = float(np.float32(r.choice([r.uniform(0, 256), float(r.integers(0, 257))])))
+        b = float(np.float32(r.uniform(0, 0.1)))
+        w = np.zeros(258, np.float32)
+        nw = lib.sponge(a, b, w.ctypes.data, len(w))
+        ref = O.sponge_table(a, b, prec=32)
+        assert nw == len(ref), (a, nw, len(ref))
+        np.testing.assert_array_equal(w[:nw], ref)
+        assert (w[nw:max(nw, 1)] == 1).all()
